@@ -1,0 +1,194 @@
+// Internal declarations shared by the library's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/ascetic.h"
+
+namespace asc {
+
+// ---- errors ----------------------------------------------------------------
+// Thread-local text of the last error, printed by the CLI.
+void set_last_error(const char* fmt, ...);
+const char* last_error();
+
+#define ASC_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            ::asc::set_last_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), \
+                                  __FILE__, __LINE__);                                  \
+            return ASC_ERR_DEVICE;                                                      \
+        }                                                                               \
+    } while (0)
+
+#define ASC_TRY(expr)              \
+    do {                           \
+        int r_ = (expr);           \
+        if (r_ != ASC_OK) return r_; \
+    } while (0)
+
+// ---- timing -----------------------------------------------------------------
+inline double now_ms() {
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+// ---- host buffers -----------------------------------------------------------
+// Uninitialised host array; pinned (hipHostMalloc) or pageable (malloc).
+template <class T>
+struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    bool pinned = false;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    HostBuf(HostBuf&& o) noexcept { *this = std::move(o); }
+    HostBuf& operator=(HostBuf&& o) noexcept {
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; pinned = o.pinned;
+            o.p = nullptr; o.n = 0;
+        }
+        return *this;
+    }
+    ~HostBuf() { release(); }
+    int alloc(size_t count, bool pin = false) {
+        release();
+        if (count == 0) return ASC_OK;
+        if (pin) {
+            if (hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+                p = nullptr;
+                set_last_error("hipHostMalloc of %zu bytes failed", count * sizeof(T));
+                return ASC_ERR_NOMEM;
+            }
+        } else {
+            p = (T*)std::malloc(count * sizeof(T));
+            if (!p) {
+                set_last_error("malloc of %zu bytes failed", count * sizeof(T));
+                return ASC_ERR_NOMEM;
+            }
+        }
+        n = count;
+        pinned = pin;
+        return ASC_OK;
+    }
+    void release() {
+        if (p) {
+            if (pinned) (void)hipHostFree(p); else std::free(p);
+        }
+        p = nullptr;
+        n = 0;
+    }
+    T& operator[](size_t i) { return p[i]; }
+    const T& operator[](size_t i) const { return p[i]; }
+};
+
+// Host CSR: offsets with sentinel (V+1), edge words (E, or 2E for {dst,w}).
+struct HostCSR {
+    uint64_t V = 0, E = 0;
+    int kind = ASC_KIND_CSR;
+    int offset_bytes = 8;   // width of the offsets in the source (file / view)
+    HostBuf<uint64_t> off;
+    HostBuf<uint32_t> edges;
+    HostBuf<uint32_t> outdeg;
+    int edge_words() const { return kind == ASC_KIND_WCSR ? 2 : 1; }
+    size_t edge_bytes() const { return 4u * (size_t)edge_words(); }
+};
+
+int read_graph_file(const char* path, int format, HostCSR& out);
+int write_graph_file(const char* path, int format, const asc_csr_view* v);
+int csr_from_view(const asc_csr_view* v, HostCSR& out);
+int validate_csr(const HostCSR& h);
+int generate_graph(const asc_gen_params* p, HostCSR& out);
+
+// ---- thread pool ------------------------------------------------------------
+class ThreadPool {
+public:
+    explicit ThreadPool(int n);
+    ~ThreadPool();
+    int size() const { return (int)workers_.size() + 1; }
+    // Runs fn(t, nthreads) on every thread (the caller is thread 0) and waits.
+    void run(const std::function<void(int, int)>& fn);
+    // Splits [0, n) into contiguous ranges, one per thread.
+    void parallel_for(uint64_t n, const std::function<void(uint64_t, uint64_t)>& fn);
+
+private:
+    void loop(int id);
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    const std::function<void(int, int)>* job_ = nullptr;
+    uint64_t gen_ = 0;
+    int pending_ = 0;
+    bool stop_ = false;
+};
+int hardware_threads();
+// Process-wide pool for host-side preprocessing (generation, loading).
+ThreadPool& host_pool();
+
+}  // namespace asc
+
+// ---- opaque handles ---------------------------------------------------------
+struct asc_ctx {
+    asc_config cfg{};
+    int device = 0;
+    int num_cus = 0;
+    uint64_t total_mem = 0;
+    hipStream_t s_main = nullptr;   // kernels
+    hipStream_t s_copy = nullptr;   // host<->device transfers of on-demand parts
+    asc::ThreadPool* pool = nullptr;  // gather threads
+};
+
+namespace asc {
+struct TravState;
+struct PrState;
+}
+
+struct asc_graph {
+    asc_ctx* ctx = nullptr;
+    asc::HostCSR h;
+    int off_bytes = 8;              // device offset width
+    // static / on-demand plan
+    uint64_t static_edges = 0;      // edges [0, static_edges) resident on device
+    uint64_t static_vertices = 0;   // vertices [0, static_vertices) are static
+    uint64_t part_edges = 0;        // capacity of each on-demand part buffer
+    double advise_k = 1.0;
+    // device copies
+    void* d_off = nullptr;          // OffT[V+1]
+    void* d_edges = nullptr;        // EdgeT[static_edges]
+    uint32_t* d_outdeg = nullptr;   // PR only
+    uint64_t device_bytes = 0;
+    // pinned mirror of the on-demand edges for zero-copy streaming
+    asc::HostBuf<uint32_t> zc_tail;
+    // tunables
+    int heavy_threshold = 0;
+    int pr_group = 0;
+    int sssp_delta = 0;
+    int serve_lanes = 64;
+    uint32_t max_degree = 0, max_degree_vertex = 0;
+    asc::TravState* trav = nullptr;
+    asc::PrState* pr = nullptr;
+};
+
+namespace asc {
+int graph_upload(asc_graph* g);       // device copies after a (re)plan
+void graph_free_device(asc_graph* g);
+void trav_free(TravState* t);
+void pr_free(PrState* p);
+int effective_heavy_threshold(const asc_graph* g);
+}  // namespace asc

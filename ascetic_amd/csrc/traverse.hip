@@ -1,0 +1,658 @@
+// Frontier traversal on gfx950: BFS, SSSP and CC (min-label propagation).
+//
+// Reference path: CalculateOpt.cu:14-545 (bfs_opt / cc_opt / sssp_opt), the
+// relax kernels gpu_kernels.cuh:345-532 and the frontier bookkeeping
+// gpu_kernels.cu:111-122, gpu_kernels.cuh:399-438 plus Thrust scans.
+//
+// Per iteration ("level") the reference classifies the active flags into a
+// static and an overload list with several O(V) passes and host syncs.  Here
+// the relax kernels themselves append improved vertices to two device queues
+// (resident: v < static_vertices, on-demand: the rest) through per-wave LDS
+// staging, so a level costs O(frontier edges) and one host sync:
+//   k_expand_local  one workgroup per 256 frontier vertices; degrees below the
+//                   heavy threshold are expanded edge-balanced inside the
+//                   workgroup (LDS prefix scan + per-lane binary search), so
+//                   consecutive lanes read consecutive edges of a vertex;
+//                   heavier vertices are appended to a grid-wide list.
+//   k_expand_heavy  grid-cooperative, edge-balanced over the heavy list.
+//   k_expand_part   edge-balanced over one streamed part of on-demand edges
+//                   (or over mapped pinned host memory in zero-copy mode).
+// Relaxation is check-then-atomicMin (gpu_kernels.cuh:390-392); a vertex is
+// queued only when its atomic returned a larger value, and once per level
+// (BFS: visited bitmap; SSSP/CC: per-vertex level stamp).
+#include <algorithm>
+
+#include "device_common.hpp"
+#include "internal.hpp"
+#include "traverse.hpp"
+
+namespace asc {
+
+// ---------------------------------------------------------------------------
+// device code
+// ---------------------------------------------------------------------------
+
+template <int ALGO>
+__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
+    return ALGO == ALG_SSSP ? sv + w : sv;  // BFS passes level + 2 as sv
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ bool relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
+    if (ALGO == ALG_BFS) {
+        const uint32_t bit = 1u << (v & 31);
+        uint32_t* word = a.visited + (v >> 5);
+        if (*word & bit) return false;
+        if (atomicOr(word, bit) & bit) return false;
+        a.val[v] = cand;
+        return true;
+    } else {
+        if (cand >= a.val[v]) return false;
+        const uint32_t old = atomicMin(a.val + v, cand);
+        if (old <= cand) return false;
+        return atomicMax(a.stamp + v, level + 1) < level + 1;
+    }
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    __shared__ uint32_t s_pref[kBlock];
+    __shared__ OffT s_beg[kBlock];
+    __shared__ uint32_t s_sv[kBlock];
+    __shared__ uint32_t s_tmp[kWavesPerBlock];
+    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
+
+    Slot* cur = a.slots + (level & 3);
+    Slot* nxt = a.slots + ((level + 1) & 3);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Slot* z = a.slots + ((level + 2) & 3);
+        z->n_res = 0;
+        z->n_od = 0;
+        z->heavy = 0;
+    }
+    const uint32_t n = cur->n_res;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
+    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
+    unsigned long long my_edges = 0, my_verts = 0;
+
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t deg = 0, sv = 0;
+        OffT beg = 0;
+        if (i < n) {
+            const uint32_t u = a.q_in[i];
+            beg = a.off[u];
+            const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+            sv = (ALGO == ALG_BFS) ? level + 2 : a.val[u];
+            if (d >= a.heavy_thr) {
+                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
+                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
+                a.heavy_u[idx] = u;
+                a.heavy_sv[idx] = sv;
+                a.heavy_base[idx] = p & kHeavyMask;
+            } else {
+                deg = (uint32_t)d;
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = block_scan_excl(deg, s_tmp, total);
+        s_pref[tid] = pre;
+        s_beg[tid] = beg;
+        s_sv[tid] = sv;
+        __syncthreads();
+        const uint32_t cnt = min(n - base, (uint32_t)kBlock);
+        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
+            const uint32_t e = eb + lane;
+            bool push = false;
+            uint32_t v = 0;
+            if (e < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kBlock / 2; step > 0; step >>= 1)
+                    if (k + step < cnt && s_pref[k + step] <= e) k += step;
+                const EdgeT ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
+                v = edge_dst(ed);
+                push = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+            }
+            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
+            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+        }
+        my_edges += total;
+        my_verts += cnt;
+        __syncthreads();
+    }
+    st_res.flush(a.q_out, &nxt->n_res);
+    st_od.flush(a.od_out, &nxt->n_od);
+    if (tid == 0 && my_verts) {
+        atomicAdd(&a.stats->edges_static, my_edges);
+        atomicAdd(&a.stats->verts, my_verts);
+    }
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    constexpr int kWin = 8;  // a tile spans <= kTile / kMinHeavy + 1 heavy entries
+    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
+    __shared__ unsigned long long s_hb[kWin];
+    __shared__ OffT s_hoff[kWin];
+    __shared__ uint32_t s_hsv[kWin];
+    __shared__ uint32_t s_k0;
+
+    Slot* cur = a.slots + (level & 3);
+    Slot* nxt = a.slots + ((level + 1) & 3);
+    const unsigned long long h = cur->heavy;
+    const uint32_t nh = (uint32_t)(h >> kHeavyShift);
+    const uint64_t total = h & kHeavyMask;
+    if (nh == 0) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
+    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
+    unsigned long long my_edges = 0;
+
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
+        if (tid == 0) {
+            uint32_t lo = 0, hi = nh - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+            }
+            s_k0 = lo;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k0;
+        const uint32_t nwin = min(nh - k0, (uint32_t)kWin);
+        if (tid < nwin) {
+            s_hb[tid] = a.heavy_base[k0 + tid];
+            s_hoff[tid] = a.off[a.heavy_u[k0 + tid]];
+            s_hsv[tid] = a.heavy_sv[k0 + tid];
+        }
+        __syncthreads();
+        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
+        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
+            const uint32_t rr = r + lane;
+            bool push = false;
+            uint32_t v = 0;
+            if (rr < tlen) {
+                const uint64_t e = t0 + rr;
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
+                    if (j < nwin && s_hb[j] <= e) k = j;
+                const EdgeT ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
+                v = edge_dst(ed);
+                push = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
+            }
+            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
+            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+        }
+        my_edges += tlen;
+        __syncthreads();
+    }
+    st_res.flush(a.q_out, &nxt->n_res);
+    st_od.flush(a.od_out, &nxt->n_od);
+    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_static, my_edges);
+}
+
+// Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
+// the on-demand list (list[k] owns [prefix[k], prefix[k+1]); no zero-degree
+// entries).  ZC = false: edges come from the device part buffer holding
+// [lo, hi).  ZC = true: from mapped pinned host memory (tail = edges from
+// static_edges on).
+template <int ALGO, class OffT, class EdgeT, bool ZC>
+__global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p) {
+    __shared__ unsigned long long s_p[kTile];
+    __shared__ uint32_t s_sv[kTile];
+    __shared__ unsigned long long s_src[ZC ? kTile : 1];
+    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
+    __shared__ uint32_t s_k[2];
+
+    Slot* nxt = a.slots + ((level + 1) & 3);
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
+    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
+    unsigned long long my_edges = 0;
+
+    for (uint64_t t0 = p.lo + (uint64_t)blockIdx.x * kTile; t0 < p.hi; t0 += (uint64_t)gridDim.x * kTile) {
+        const uint64_t t1 = min(t0 + (uint64_t)kTile, p.hi);
+        if (tid < 2) {
+            const uint64_t key = tid == 0 ? t0 : t1 - 1;
+            uint32_t lo = 0, hi = p.n - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (p.prefix[mid] <= key) lo = mid; else hi = mid - 1;
+            }
+            s_k[tid] = lo;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k[0];
+        const uint32_t m = s_k[1] - k0 + 1;
+        for (uint32_t j = tid; j < m; j += kBlock) {
+            const uint32_t u = p.list[k0 + j];
+            s_p[j] = p.prefix[k0 + j];
+            s_sv[j] = (ALGO == ALG_BFS) ? level + 2 : a.val[u];
+            if (ZC) s_src[j] = (unsigned long long)a.off[u] - p.tail_base;
+        }
+        __syncthreads();
+        const uint32_t tlen = (uint32_t)(t1 - t0);
+        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
+            const uint32_t rr = r + lane;
+            bool push = false;
+            uint32_t v = 0;
+            if (rr < tlen) {
+                const uint64_t e = t0 + rr;
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kTile / 2; step > 0; step >>= 1)
+                    if (k + step < m && s_p[k + step] <= e) k += step;
+                EdgeT ed;
+                if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
+                else ed = p.part[e - p.lo];
+                v = edge_dst(ed);
+                push = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+            }
+            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
+            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+        }
+        my_edges += tlen;
+        __syncthreads();
+    }
+    st_res.flush(a.q_out, &nxt->n_res);
+    st_od.flush(a.od_out, &nxt->n_od);
+    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_part, my_edges);
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Level-0 frontier for BFS / SSSP: the source alone (GraphMeta.cuh:432-446).
+__global__ void k_init_source(uint32_t* val, uint32_t* visited, uint32_t* q, uint32_t* od, Slot* slots, uint32_t src,
+                              uint32_t vs) {
+    val[src] = 1;
+    if (visited) visited[src >> 5] |= 1u << (src & 31);
+    if (src < vs) {
+        q[0] = src;
+        slots[0].n_res = 1;
+    } else {
+        od[0] = src;
+        slots[0].n_od = 1;
+    }
+}
+
+// CC: every vertex starts with its own id and active (GraphMeta.cuh:448-452).
+__global__ void k_init_cc(uint32_t* val, uint32_t* q, uint32_t* od, Slot* slots, uint32_t V, uint32_t vs) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        val[v] = v;
+        if (v < vs) q[v] = v; else od[v - vs] = v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        slots[0].n_res = vs;
+        slots[0].n_od = V - vs;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+void trav_free(TravState* t) {
+    if (!t) return;
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(t->val); f(t->stamp); f(t->visited);
+    f(t->q[0]); f(t->q[1]); f(t->od[0]); f(t->od[1]);
+    f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
+    f(t->od_list); f(t->od_prefix); f(t->slots); f(t->stats);
+    f(t->part[0]); f(t->part[1]);
+    for (int i = 0; i < 2; i++) {
+        if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
+        if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
+    }
+    if (t->ev_meta) (void)hipEventDestroy(t->ev_meta);
+    for (auto e : t->ev_pool) (void)hipEventDestroy(e);
+    delete t;
+}
+
+static int trav_ensure(asc_graph* g) {
+    const uint64_t V = g->h.V;
+    const size_t EB = g->h.edge_bytes();
+    TravState* t = g->trav;
+    const bool need_parts = g->static_vertices < V;
+    if (t && t->V == V && (!need_parts || (t->part[0] && t->part_edges >= g->part_edges))) return ASC_OK;
+    if (t) trav_free(t);
+    t = new TravState();
+    g->trav = t;
+    t->V = V;
+    ASC_HIP(hipSetDevice(g->ctx->device));
+    ASC_HIP(hipMalloc(&t->val, 4 * V));
+    ASC_HIP(hipMalloc(&t->stamp, 4 * V));
+    ASC_HIP(hipMalloc(&t->visited, 4 * ((V + 31) / 32)));
+    for (int i = 0; i < 2; i++) {
+        ASC_HIP(hipMalloc(&t->q[i], 4 * V));
+        ASC_HIP(hipMalloc(&t->od[i], 4 * V));
+    }
+    ASC_HIP(hipMalloc(&t->heavy_u, 4 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&t->heavy_sv, 4 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&t->heavy_base, 8 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&t->slots, sizeof(Slot) * 4));
+    ASC_HIP(hipMalloc(&t->stats, sizeof(DevStats)));
+    ASC_TRY(t->h_slot.alloc(4, true));
+    ASC_TRY(t->h_stats.alloc(1, true));
+    if (need_parts) {
+        ASC_HIP(hipMalloc(&t->od_list, 4 * V));
+        ASC_HIP(hipMalloc(&t->od_prefix, 8 * (V + 1)));
+        ASC_TRY(t->h_od.alloc(V, true));
+        ASC_TRY(t->h_list.alloc(V, true));
+        ASC_TRY(t->h_prefix.alloc(V + 1, true));
+        t->part_edges = g->part_edges;
+        for (int i = 0; i < 2; i++) {
+            if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER) {
+                ASC_HIP(hipMalloc(&t->part[i], EB * t->part_edges));
+                ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
+            }
+            ASC_HIP(hipEventCreateWithFlags(&t->ev_copy[i], hipEventDisableTiming));
+            ASC_HIP(hipEventCreateWithFlags(&t->ev_kern[i], hipEventDisableTiming));
+        }
+        ASC_HIP(hipEventCreateWithFlags(&t->ev_meta, hipEventDisableTiming));
+    }
+    return ASC_OK;
+}
+
+static hipEvent_t pooled_event(TravState* t, size_t i) {
+    while (t->ev_pool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        t->ev_pool.push_back(e);
+    }
+    return t->ev_pool[i];
+}
+
+int effective_heavy_threshold(const asc_graph* g) {
+    uint64_t thr = g->heavy_threshold > 0 ? (uint64_t)g->heavy_threshold : 2048;
+    const uint64_t floor_cap = (g->h.E + kHeavyCap - 1) / kHeavyCap;  // list can never overflow
+    thr = std::max<uint64_t>(thr, floor_cap);
+    thr = std::max<uint64_t>(thr, (uint64_t)kMinHeavy);
+    thr = std::min<uint64_t>(thr, uint64_t(1) << 22);
+    return (int)thr;
+}
+
+// Copies the adjacency of list entries overlapping [lo, hi) of the
+// concatenated on-demand edge list into `dst` (the reference's
+// fillEdgeArrByMultiThread, GraphMeta.cuh:469-514, cut to one part).
+static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long long* prefix, uint64_t m,
+                        uint64_t lo, uint64_t hi, char* dst) {
+    const size_t EB = g->h.edge_bytes();
+    const char* src = (const char*)g->h.edges.p;
+    const uint64_t* off = g->h.off.p;
+    g->ctx->pool->parallel_for(hi - lo, [&](uint64_t b, uint64_t e) {
+        uint64_t a = lo + b;
+        const uint64_t end = lo + e;
+        uint64_t k = (uint64_t)(std::upper_bound(prefix, prefix + m, (unsigned long long)a) - prefix) - 1;
+        while (a < end) {
+            const uint64_t kend = std::min<uint64_t>(prefix[k + 1], end);
+            const uint64_t from = off[list[k]] + (a - prefix[k]);
+            std::memcpy(dst + (a - lo) * EB, src + from * EB, (kend - a) * EB);
+            a = kend;
+            k++;
+        }
+    });
+}
+
+template <int ALGO, class OffT, class EdgeT>
+static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, uint32_t no, int grid,
+                        RunCounters& rc) {
+    asc_ctx* ctx = g->ctx;
+    TravState* t = g->trav;
+    const double t_begin = now_ms();
+    // 1. on-demand list of this level (written by the previous level's kernels,
+    //    which completed before this call)
+    ASC_HIP(hipMemcpyAsync(t->h_od.p, t->od[level & 1], 4 * (size_t)no, hipMemcpyDeviceToHost, ctx->s_copy));
+    ASC_HIP(hipStreamSynchronize(ctx->s_copy));
+    // 2. drop zero-degree entries, u64 prefix of degrees
+    const uint64_t* off = g->h.off.p;
+    const int nt = ctx->pool->size();
+    std::vector<uint64_t> cnt(nt + 1, 0), esum(nt + 1, 0);
+    const uint64_t chunk = (no + nt - 1) / nt;
+    ctx->pool->run([&](int tt, int) {
+        uint64_t b = std::min<uint64_t>(no, tt * chunk), e = std::min<uint64_t>(no, b + chunk), c = 0, s = 0;
+        for (uint64_t i = b; i < e; i++) {
+            uint64_t d = off[t->h_od[i] + 1] - off[t->h_od[i]];
+            c += d > 0;
+            s += d;
+        }
+        cnt[tt + 1] = c;
+        esum[tt + 1] = s;
+    });
+    for (int i = 0; i < nt; i++) {
+        cnt[i + 1] += cnt[i];
+        esum[i + 1] += esum[i];
+    }
+    ctx->pool->run([&](int tt, int) {
+        uint64_t b = std::min<uint64_t>(no, tt * chunk), e = std::min<uint64_t>(no, b + chunk);
+        uint64_t c = cnt[tt], s = esum[tt];
+        for (uint64_t i = b; i < e; i++) {
+            const uint32_t u = t->h_od[i];
+            uint64_t d = off[u + 1] - off[u];
+            if (!d) continue;
+            t->h_list[c] = u;
+            t->h_prefix[c] = s;
+            c++;
+            s += d;
+        }
+    });
+    const uint64_t m = cnt[nt], T = esum[nt];
+    t->h_prefix[m] = T;
+    rc.pre_ms += now_ms() - t_begin;
+    if (m == 0) return ASC_OK;
+    ASC_HIP(hipMemcpyAsync(t->od_list, t->h_list.p, 4 * m, hipMemcpyHostToDevice, ctx->s_copy));
+    ASC_HIP(hipMemcpyAsync(t->od_prefix, t->h_prefix.p, 8 * (m + 1), hipMemcpyHostToDevice, ctx->s_copy));
+    ASC_HIP(hipEventRecord(t->ev_meta, ctx->s_copy));
+    ASC_HIP(hipStreamWaitEvent(ctx->s_main, t->ev_meta, 0));
+
+    PartArgs<EdgeT> p{};
+    p.list = t->od_list;
+    p.prefix = t->od_prefix;
+    p.n = (uint32_t)m;
+    p.tail_base = g->static_edges;
+    const size_t EB = g->h.edge_bytes();
+    if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
+        void* dptr = nullptr;
+        ASC_HIP(hipHostGetDevicePointer(&dptr, g->zc_tail.p, 0));
+        p.zc = (const EdgeT*)dptr;
+        p.lo = 0;
+        p.hi = T;
+        hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, ctx->s_main, a,
+                           level, p);
+        ASC_HIP(hipGetLastError());
+        rc.parts += 1;
+    } else {
+        const uint64_t PE = t->part_edges;
+        const uint64_t np = (T + PE - 1) / PE;
+        for (uint64_t q = 0; q < np; q++) {
+            const int b = (int)(q & 1);
+            const uint64_t lo = q * PE, hi = std::min(T, lo + PE);
+            if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_copy[b]));  // staging b drained
+            const double tg = now_ms();
+            gather_part(g, t->h_list.p, t->h_prefix.p, m, lo, hi, t->stage[b].p);
+            rc.fill_ms += now_ms() - tg;
+            if (q >= 2) ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
+            ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[b].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
+            ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+            ASC_HIP(hipStreamWaitEvent(ctx->s_main, t->ev_copy[b], 0));
+            p.part = (const EdgeT*)t->part[b];
+            p.lo = lo;
+            p.hi = hi;
+            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0, ctx->s_main,
+                               a, level, p);
+            ASC_HIP(hipGetLastError());
+            ASC_HIP(hipEventRecord(t->ev_kern[b], ctx->s_main));
+        }
+        rc.parts += (uint32_t)np;
+    }
+    rc.streamed_edges += T;
+    rc.overload_ms += now_ms() - t_begin;
+    return ASC_OK;
+}
+
+template <int ALGO, class OffT, class EdgeT>
+static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) {
+    asc_ctx* ctx = g->ctx;
+    ASC_TRY(trav_ensure(g));
+    TravState* t = g->trav;
+    const uint64_t V = g->h.V;
+    const uint32_t vs = (uint32_t)g->static_vertices;
+    hipStream_t s = ctx->s_main;
+    ASC_HIP(hipSetDevice(ctx->device));
+    const int fill_grid = ctx->num_cus * 4;
+    const int grid = ctx->num_cus * kBlocksPerCU;
+
+    // initial state
+    ASC_HIP(hipMemsetAsync(t->slots, 0, sizeof(Slot) * 4, s));
+    ASC_HIP(hipMemsetAsync(t->stats, 0, sizeof(DevStats), s));
+    if (ALGO == ALG_CC) {
+        hipLaunchKernelGGL(k_init_cc, dim3(fill_grid), dim3(256), 0, s, t->val, t->q[0], t->od[0], t->slots,
+                           (uint32_t)V, vs);
+    } else {
+        hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->val, V, (uint32_t)(V + 1));
+        if (ALGO == ALG_BFS)
+            hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->visited, (V + 31) / 32, 0u);
+        hipLaunchKernelGGL(k_init_source, dim3(1), dim3(1), 0, s, t->val, ALGO == ALG_BFS ? t->visited : nullptr,
+                           t->q[0], t->od[0], t->slots, src, vs);
+    }
+    if (ALGO != ALG_BFS) hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->stamp, V, 0u);
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipStreamSynchronize(s));
+
+    TravArgs<OffT, EdgeT> a{};
+    a.off = (const OffT*)g->d_off;
+    a.edges = (const EdgeT*)g->d_edges;
+    a.val = t->val;
+    a.stamp = t->stamp;
+    a.visited = t->visited;
+    a.slots = t->slots;
+    a.heavy_u = t->heavy_u;
+    a.heavy_sv = t->heavy_sv;
+    a.heavy_base = t->heavy_base;
+    a.stats = t->stats;
+    a.vs = vs;
+    a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
+
+    uint32_t nr, no;
+    if (ALGO == ALG_CC) {
+        nr = vs;
+        no = (uint32_t)(V - vs);
+    } else {
+        nr = src < vs ? 1 : 0;
+        no = src < vs ? 0 : 1;
+    }
+    RunCounters rc{};
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kev;
+    const double t0 = now_ms();
+    uint32_t level = 0;
+    for (;; level++) {
+        if (nr == 0 && no == 0) break;
+        rc.node_sum += (uint64_t)nr + no;
+        a.q_in = t->q[level & 1];
+        a.q_out = t->q[(level + 1) & 1];
+        a.od_out = t->od[(level + 1) & 1];
+        hipEvent_t e0 = pooled_event(t, 2 * level), e1 = pooled_event(t, 2 * level + 1);
+        if (!e0 || !e1) return ASC_ERR_DEVICE;
+        ASC_HIP(hipEventRecord(e0, s));
+        hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        ASC_HIP(hipGetLastError());
+        ASC_HIP(hipEventRecord(e1, s));
+        kev.emplace_back(e0, e1);
+        if (no > 0) ASC_TRY((stream_level<ALGO, OffT, EdgeT>(g, a, level, no, grid, rc)));
+        ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + ((level + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipStreamSynchronize(s));
+        nr = t->h_slot[0].n_res;
+        no = t->h_slot[0].n_od;
+    }
+    const double t1 = now_ms();
+    double kms = 0;
+    for (auto& pr : kev) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kms += ms;
+    }
+    ASC_HIP(hipMemcpy(t->h_stats.p, t->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
+    if (out) ASC_HIP(hipMemcpy(out, t->val, 4 * V, hipMemcpyDeviceToHost));
+
+    if (st) {
+        std::memset(st, 0, sizeof(*st));
+        st->total_ms = t1 - t0;
+        st->pre_ms = rc.pre_ms;
+        st->static_ms = kms;
+        st->kernel_ms = kms;
+        st->overload_ms = rc.overload_ms;
+        st->fill_ms = rc.fill_ms;
+        st->iterations = level;
+        st->parts = rc.parts;
+        st->node_sum = rc.node_sum;
+        st->static_edges_traversed = t->h_stats[0].edges_static;
+        st->edges_traversed = t->h_stats[0].edges_static + t->h_stats[0].edges_part;
+        st->streamed_edges = rc.streamed_edges;
+        st->streamed_bytes = rc.streamed_edges * g->h.edge_bytes();
+        st->vertices_processed = t->h_stats[0].verts;
+        st->host_cores = hardware_threads();
+    }
+    if (out) {
+        // reached vertices and their degree sum (Graph500 TEPS numerator)
+        const uint32_t unreached = (uint32_t)(V + 1);
+        std::atomic<uint64_t> reached{0}, dsum{0};
+        const uint64_t* off = g->h.off.p;
+        const int conv = ctx->cfg.value_convention;
+        host_pool().parallel_for(V, [&](uint64_t b, uint64_t e) {
+            uint64_t r = 0, d = 0;
+            for (uint64_t v = b; v < e; v++) {
+                if (ALGO == ALG_CC || out[v] != unreached) {
+                    r++;
+                    d += off[v + 1] - off[v];
+                }
+                if (ALGO != ALG_CC && conv == ASC_VALUES_ZERO_BASED)
+                    out[v] = out[v] == unreached ? 0xFFFFFFFFu : out[v] - 1;
+            }
+            reached += r;
+            dsum += d;
+        });
+        if (st) {
+            st->reached = reached;
+            st->reached_degree_sum = dsum;
+        }
+    }
+    if (ctx->cfg.verbose && st) {
+        printf("totalProcess time is %.3f ms\npreProcess time is %.3f ms\nstaticProcess time is %.3f ms\n"
+               "overloadProcess time is %.3f ms\nfillProcess time is %.3f ms\nnodeSum : %llu\n"
+               "move overload size : %llu\n",
+               st->total_ms, st->pre_ms, st->static_ms, st->overload_ms, st->fill_ms,
+               (unsigned long long)st->node_sum, (unsigned long long)st->streamed_bytes);
+    }
+    return ASC_OK;
+}
+
+int traverse(asc_graph* g, int algo, uint32_t src, uint32_t* out, asc_stats* st) {
+    const bool w = g->h.kind == ASC_KIND_WCSR;
+    const bool o4 = g->off_bytes == 4;
+    switch (algo) {
+        case ALG_BFS:
+            if (w) return o4 ? run_traversal<ALG_BFS, uint32_t, WEdge>(g, src, out, st)
+                             : run_traversal<ALG_BFS, uint64_t, WEdge>(g, src, out, st);
+            return o4 ? run_traversal<ALG_BFS, uint32_t, uint32_t>(g, src, out, st)
+                      : run_traversal<ALG_BFS, uint64_t, uint32_t>(g, src, out, st);
+        case ALG_SSSP:
+            if (!w) return ASC_ERR_UNSUPPORTED;
+            return o4 ? run_traversal<ALG_SSSP, uint32_t, WEdge>(g, src, out, st)
+                      : run_traversal<ALG_SSSP, uint64_t, WEdge>(g, src, out, st);
+        case ALG_CC:
+            if (w) return o4 ? run_traversal<ALG_CC, uint32_t, WEdge>(g, src, out, st)
+                             : run_traversal<ALG_CC, uint64_t, WEdge>(g, src, out, st);
+            return o4 ? run_traversal<ALG_CC, uint32_t, uint32_t>(g, src, out, st)
+                      : run_traversal<ALG_CC, uint64_t, uint32_t>(g, src, out, st);
+    }
+    return ASC_ERR_INVALID;
+}
+
+}  // namespace asc

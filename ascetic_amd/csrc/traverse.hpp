@@ -1,0 +1,98 @@
+// Shared declarations of the frontier traversal engine.
+#pragma once
+
+#include "device_common.hpp"
+#include "internal.hpp"
+
+namespace asc {
+
+enum { ALG_BFS = 0, ALG_SSSP = 1, ALG_CC = 2 };
+
+constexpr int kStageCap = 128;                  // per-wave LDS staging entries, per queue
+constexpr int kHeavyShift = 44;                 // heavy list counter: (count << 44) | edges
+constexpr unsigned long long kHeavyMask = (1ull << kHeavyShift) - 1;
+constexpr uint32_t kHeavyCap = 1u << 20;        // heavy list entries per level
+constexpr int kTile = 1024;                     // edges per tile in edge-balanced kernels
+constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
+constexpr int kBlocksPerCU = 8;
+
+// Per-level counters, in a ring of four: level L reads slot L%4, appends to
+// slot (L+1)%4 and clears slot (L+2)%4.
+struct Slot {
+    uint32_t n_res;                 // resident frontier entries
+    uint32_t n_od;                  // on-demand frontier entries
+    unsigned long long heavy;       // heavy list (count << 44 | edges)
+    unsigned long long pad[2];
+};
+
+struct DevStats {
+    unsigned long long edges_static;
+    unsigned long long edges_part;
+    unsigned long long verts;
+    unsigned long long pad;
+};
+
+template <class OffT, class EdgeT>
+struct TravArgs {
+    const OffT* off;                // V+1 offsets
+    const EdgeT* edges;             // static edge prefix
+    uint32_t* val;
+    uint32_t* stamp;                // SSSP/CC: level a vertex was last queued for
+    uint32_t* visited;              // BFS bitmap
+    const uint32_t* q_in;
+    uint32_t* q_out;
+    uint32_t* od_out;
+    Slot* slots;
+    uint32_t* heavy_u;
+    uint32_t* heavy_sv;
+    unsigned long long* heavy_base;
+    DevStats* stats;
+    uint32_t vs;                    // vertices [0, vs) are resident
+    uint32_t heavy_thr;
+};
+
+template <class EdgeT>
+struct PartArgs {
+    const uint32_t* list;           // on-demand vertices (no zero degree)
+    const unsigned long long* prefix;  // exclusive edge starts, prefix[n] = total
+    uint32_t n;
+    unsigned long long lo, hi;      // edge range of this part
+    const EdgeT* part;              // device buffer holding edges [lo, hi)
+    const EdgeT* zc;                // zero-copy: mapped host edges from tail_base
+    unsigned long long tail_base;
+};
+
+struct RunCounters {
+    double pre_ms = 0, overload_ms = 0, fill_ms = 0;
+    uint64_t node_sum = 0, streamed_edges = 0;
+    uint32_t parts = 0;
+};
+
+struct TravState {
+    uint64_t V = 0;
+    uint32_t* val = nullptr;
+    uint32_t* stamp = nullptr;
+    uint32_t* visited = nullptr;
+    uint32_t* q[2] = {nullptr, nullptr};
+    uint32_t* od[2] = {nullptr, nullptr};
+    uint32_t* heavy_u = nullptr;
+    uint32_t* heavy_sv = nullptr;
+    unsigned long long* heavy_base = nullptr;
+    uint32_t* od_list = nullptr;
+    unsigned long long* od_prefix = nullptr;
+    Slot* slots = nullptr;
+    DevStats* stats = nullptr;
+    void* part[2] = {nullptr, nullptr};
+    uint64_t part_edges = 0;
+    HostBuf<Slot> h_slot;
+    HostBuf<DevStats> h_stats;
+    HostBuf<uint32_t> h_od, h_list;
+    HostBuf<unsigned long long> h_prefix;
+    HostBuf<char> stage[2];
+    hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr}, ev_meta = nullptr;
+    std::vector<hipEvent_t> ev_pool;
+};
+
+int traverse(asc_graph* g, int algo, uint32_t src, uint32_t* out, asc_stats* st);
+
+}  // namespace asc

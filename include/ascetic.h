@@ -1,0 +1,341 @@
+/*
+ * ascetic.h -- C ABI of the MI355X (gfx950) graph-traversal library.
+ *
+ * The library runs BFS, SSSP, connected components (min-label propagation)
+ * and PageRank over binary CSR graphs on one AMD Instinct MI355X, with the
+ * edge array split into a device-resident static prefix and an on-demand
+ * remainder that is streamed from pinned host memory each iteration.
+ *
+ * Every entry point returns 0 on success or a negative asc_status code; no
+ * entry point terminates the process.  Output buffers are caller-allocated
+ * host arrays of length V (number of vertices).  All sizes are 64-bit.
+ *
+ * Reference interfaces replaced (NKU-EmbeddedSystem/Ascetic):
+ *   bfs_opt / cc_opt / sssp_opt / pr_opt ........ CalculateOpt.cuh:11-14
+ *   GraphMeta<E>::readDataFromFile .............. GraphMeta.cuh:151-171
+ *   GraphMeta<E>::getMaxPartitionSize ........... GraphMeta.cuh:310-372
+ *   GraphMeta<E>::caculatePartInfoForEdgeList ... GraphMeta.cuh:516-578
+ *   ArgumentParser / main (the ptGraph CLI) ..... ArgumentParser.cu:3-48, main.cu:4-22
+ */
+#ifndef ASCETIC_H
+#define ASCETIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASC_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------- */
+typedef enum {
+    ASC_OK = 0,
+    ASC_ERR_INVALID = -1,   /* bad argument (null pointer, source >= V, ...) */
+    ASC_ERR_IO = -2,        /* file could not be opened / read / written     */
+    ASC_ERR_FORMAT = -3,    /* file size does not match any known layout     */
+    ASC_ERR_NOMEM = -4,     /* host or device allocation failed              */
+    ASC_ERR_DEVICE = -5,    /* a HIP call failed                             */
+    ASC_ERR_NOGPU = -6,     /* no HIP device available                       */
+    ASC_ERR_RANGE = -7,     /* size beyond a documented limit                */
+    ASC_ERR_UNSUPPORTED = -8,/* graph kind does not support this algorithm   */
+    ASC_ERR_BUSY = -9,      /* server queue full / no free query lane        */
+    ASC_ERR_STATE = -10     /* call not valid in the object's current state  */
+} asc_status;
+
+const char* asc_strerror(int code);
+int asc_abi_version(void);
+
+/* ---- on-disk / in-memory formats --------------------------------------- */
+/* Binary layouts, little-endian (reader: GraphMeta.cuh:151-171,
+ * writer: converter/converter.cpp:178-181, 296-300, 413-416):
+ *   BCSR     u64 V, u64 E, u64 off[V], u32 dst[E]
+ *   BWCSR    u64 V, u64 E, u64 off[V], {u32 dst, u32 w}[E]
+ *   BCSC     u64 V, u64 E, u32 outdeg[V], u64 inoff[V], u32 src[E]
+ *   LEGACY32 u32 V, u32 E, u32 off[V], u32 dst[E]          (bfs.cu:294-304)
+ *   LEGACY32W u32 V, u32 E, u32 off[V], {u32 dst, u32 w}[E] (sssp.cu:176-186)
+ * Offsets carry no off[V] sentinel; the last degree is E - off[V-1]
+ * (GraphMeta.cuh:226).  ASC_FMT_AUTO decides by the file size. */
+typedef enum {
+    ASC_FMT_AUTO = 0,
+    ASC_FMT_BCSR = 1,
+    ASC_FMT_BWCSR = 2,
+    ASC_FMT_BCSC = 3,
+    ASC_FMT_LEGACY32 = 4,
+    ASC_FMT_LEGACY32W = 5
+} asc_format;
+
+/* Graph kinds. */
+typedef enum {
+    ASC_KIND_CSR = 0,       /* out-CSR, unweighted (bfs, cc)                 */
+    ASC_KIND_WCSR = 1,      /* out-CSR, AoS {dst,w} (sssp; bfs/cc too)        */
+    ASC_KIND_CSC = 2        /* in-CSR + out-degree (pr)                      */
+} asc_kind;
+
+/* Value conventions of the integer outputs (bfs / sssp).
+ * REFERENCE: source = 1, unreached = V+1 (GraphMeta.cuh:432-446).
+ * ZERO_BASED: source = 0, unreached = 0xFFFFFFFF. */
+typedef enum {
+    ASC_VALUES_REFERENCE = 0,
+    ASC_VALUES_ZERO_BASED = 1
+} asc_value_convention;
+
+/* ---- context ----------------------------------------------------------- */
+typedef enum {
+    ASC_STREAM_GATHER = 0,  /* host threads gather active on-demand edges into
+                               pinned staging; double-buffered hipMemcpyAsync */
+    ASC_STREAM_ZEROCOPY = 1 /* kernels read on-demand edges from mapped pinned
+                               host memory directly                         */
+} asc_stream_mode;
+
+typedef struct {
+    int device;                  /* HIP device ordinal                        */
+    double advise_rate;          /* --adviseK: fraction of the device edge
+                                    capacity used as static region; <= 0 means
+                                    the reference's automatic rule
+                                    (GraphMeta.cuh:326-339)                   */
+    uint64_t edge_budget_bytes;  /* > 0: static region limited to this many
+                                    bytes of edge records (overrides
+                                    advise_rate); 0: derived from free memory */
+    uint64_t part_bytes;         /* bytes of each of the two on-demand part
+                                    buffers; 0 = 256 MiB                      */
+    uint64_t device_mem_limit;   /* > 0: pretend the device has this much free
+                                    memory when sizing (testing); 0 = query   */
+    int gather_threads;          /* host gather threads; 0 = min(16, cores)   */
+    int stream_mode;             /* asc_stream_mode                           */
+    int value_convention;        /* asc_value_convention                      */
+    int heavy_threshold;         /* degree at/above which a frontier vertex is
+                                    expanded grid-cooperatively; 0 = tuned    */
+    int pr_group;                /* lanes per vertex in the PageRank pull
+                                    (4,8,16,32,64); 0 = tuned                 */
+    int sssp_delta;              /* SSSP bucket width (near-far); 0 = tuned,
+                                    <0 = plain frontier Bellman-Ford          */
+    int verbose;                 /* print per-phase timings like the reference */
+} asc_config;
+
+typedef struct asc_ctx asc_ctx;
+typedef struct asc_graph asc_graph;
+
+void asc_config_init(asc_config* cfg);
+int asc_ctx_create(const asc_config* cfg, asc_ctx** out);
+int asc_ctx_destroy(asc_ctx* ctx);
+/* Number of visible HIP devices (0 when none); never fails. */
+int asc_device_count(void);
+/* Blocks until all work the library queued on the context's device ended. */
+int asc_ctx_synchronize(asc_ctx* ctx);
+
+/* ---- graphs ------------------------------------------------------------ */
+/* Host view of a CSR/CSC graph.  Exactly one of off64/off32 is set; it holds
+ * V entries (no sentinel) or V+1 entries (has_sentinel = 1).  For ASC_KIND_CSR
+ * and ASC_KIND_CSC `edges` points to u32 ids; for ASC_KIND_WCSR to E pairs
+ * {u32 dst, u32 w}.  For ASC_KIND_CSC out_degree holds V u32 values. */
+typedef struct {
+    uint64_t num_vertices;
+    uint64_t num_edges;
+    int kind;                    /* asc_kind                                 */
+    int has_sentinel;
+    const uint64_t* off64;
+    const uint32_t* off32;
+    const uint32_t* edges;
+    const uint32_t* out_degree;
+} asc_csr_view;
+
+/* Reads a graph file (GraphMeta.cuh:151-171).  The file is validated against
+ * its header by size; offsets must be non-decreasing and <= E, ids < V. */
+int asc_graph_load(asc_ctx* ctx, const char* path, int format, asc_graph** out);
+/* Builds a graph from caller arrays (they are copied; the caller keeps them). */
+int asc_graph_from_csr(asc_ctx* ctx, const asc_csr_view* view, asc_graph** out);
+int asc_graph_destroy(asc_graph* g);
+
+typedef struct {
+    uint64_t num_vertices;
+    uint64_t num_edges;
+    int kind;
+    int offset_bytes;            /* 4 or 8: width of offsets on the device    */
+    uint64_t static_edges;       /* edges in the device-resident prefix       */
+    uint64_t static_vertices;    /* vertices [0, static_vertices) are static  */
+    uint64_t ondemand_edges;     /* edges left to stream                      */
+    uint64_t part_edges;         /* edges per on-demand part buffer           */
+    uint64_t device_bytes;       /* device memory the graph holds             */
+    uint32_t max_degree;
+    uint32_t max_degree_vertex;
+} asc_graph_info;
+int asc_graph_get_info(const asc_graph* g, asc_graph_info* info);
+
+/* Writes a graph file in one of the binary layouts. */
+int asc_write_graph(const char* path, int format, const asc_csr_view* view);
+
+/* Re-plans the static / on-demand split of a loaded graph with a new byte
+ * budget (0 = automatic rule) or advise rate; uploads the new static prefix. */
+int asc_graph_replan(asc_graph* g, uint64_t edge_budget_bytes, double advise_rate);
+
+/* ---- per-run statistics (phase names follow CalculateOpt.cu:30-36) ------ */
+typedef struct {
+    double total_ms;             /* first iteration start .. last end         */
+    double pre_ms;               /* host-side frontier bookkeeping            */
+    double static_ms;            /* resident relax kernels (HIP events)       */
+    double overload_ms;          /* on-demand phase wall time                 */
+    double overload_move_ms;     /* H2D copies of on-demand parts (events)    */
+    double fill_ms;              /* host gather of on-demand edges            */
+    uint32_t iterations;
+    uint32_t parts;              /* on-demand parts shipped                   */
+    uint64_t node_sum;           /* sum over iterations of active vertices    */
+    uint64_t edges_traversed;    /* edges relaxed (static + on-demand)        */
+    uint64_t static_edges_traversed;
+    uint64_t streamed_edges;     /* on-demand edges shipped over the link     */
+    uint64_t streamed_bytes;
+    uint64_t vertices_processed; /* frontier entries expanded                 */
+    uint64_t reached;            /* vertices with a final value != unreached  */
+    uint64_t reached_degree_sum; /* sum of out-degrees of reached vertices    */
+    double kernel_ms;            /* sum of relax-kernel durations (events)    */
+    double pr_residual;          /* PageRank: last iteration's L1 residual    */
+    int host_cores;
+} asc_stats;
+
+/* ---- algorithms (CalculateOpt.cu:14-721) ------------------------------- */
+int asc_bfs(asc_graph* g, uint32_t source, uint32_t* level_out, asc_stats* st);
+int asc_sssp(asc_graph* g, uint32_t source, uint32_t* dist_out, asc_stats* st);
+int asc_cc(asc_graph* g, uint32_t* label_out, asc_stats* st);
+
+typedef enum {
+    ASC_PR_NORMALISED = 0,       /* r = (1-d)/V + d*(sum r/outdeg + dangling/V),
+                                    fixed iterations, FP32 ranks, FP64 residual */
+    ASC_PR_REFERENCE = 1         /* 0.15 + 0.85*sum, active set with threshold,
+                                    FP64 (gpu_kernels.cuh:533-594)             */
+} asc_pr_mode;
+
+typedef struct {
+    int mode;                    /* asc_pr_mode                               */
+    double damping;              /* 0.85                                      */
+    int iterations;              /* normalised mode: fixed count (20)         */
+    double threshold;            /* reference mode: 0.001                     */
+    int max_iterations;          /* reference mode safety cap (0 = 10000)     */
+} asc_pr_opts;
+void asc_pr_opts_init(asc_pr_opts* o);
+
+/* rank_f32: V floats (normalised mode); rank_f64: V doubles (reference mode);
+ * residual_out (optional): per-iteration L1 residuals, `iterations` entries. */
+int asc_pr(asc_graph* g, const asc_pr_opts* opts, float* rank_f32, double* rank_f64,
+           double* residual_out, asc_stats* st);
+
+/* ---- static / on-demand planning (GraphMeta.cuh:310-372, 516-578) ------- */
+typedef struct {
+    uint64_t total_device_edges; /* (free - param_size*4*V) / edge_size        */
+    double advise_k;             /* rate actually used                        */
+    uint64_t static_edges;       /* static prefix length in edges             */
+    uint64_t static_vertices;    /* vertices whose adjacency is all static    */
+    uint64_t ondemand_edges;     /* E - sum of static degrees                 */
+    uint64_t part_edges;         /* on-demand part capacity in edges          */
+} asc_plan;
+
+/* Pure host function: plans the split for a graph with the given offsets.
+ * free_bytes: device memory available; param_size: 13 (bfs, cc), 15 (sssp),
+ * 17 (pr) as in CalculateOpt.cu:20,214,389,552. */
+int asc_plan_partition(uint64_t V, uint64_t E, const uint64_t* off64, uint64_t edge_size,
+                       uint64_t free_bytes, int param_size, double advise_rate,
+                       uint64_t edge_budget_bytes, uint64_t part_bytes, asc_plan* out);
+
+/* Edge-granular part split of an on-demand list: given prefix[0..n] (u64
+ * exclusive starts of each active vertex's adjacency in the gathered list,
+ * prefix[n] = total) and a part capacity in edges, writes the edge start of
+ * each part into part_starts (n_parts+1 entries, last = total).  Unlike the
+ * reference's vertex-granular split, a vertex larger than a part spans
+ * several parts.  Returns the number of parts via *n_parts. */
+int asc_split_parts(const uint64_t* prefix, uint64_t n, uint64_t part_edges,
+                    uint64_t* part_starts, uint64_t cap, uint64_t* n_parts);
+
+/* ---- synthetic graphs -------------------------------------------------- */
+typedef struct {
+    int type;                    /* 0 = R-MAT, 1 = Erdos-Renyi                */
+    int scale;                   /* R-MAT: V = 2^scale                        */
+    uint64_t num_vertices;       /* ER: V                                     */
+    uint64_t num_samples;        /* ER: undirected pairs drawn; R-MAT: 0 means
+                                    edge_factor * V                           */
+    int edge_factor;             /* R-MAT                                     */
+    double a, b, c;              /* R-MAT quadrant probabilities (d = 1-a-b-c) */
+    uint64_t seed;
+    int symmetrize;              /* add (v,u) for every (u,v)                 */
+    int weighted;                /* AoS {dst,w}; w uniform in [wmin,wmax],
+                                    w(u,v) = w(v,u)                           */
+    uint32_t wmin, wmax;
+    int offset_bytes;            /* 4 or 8: offsets the graph keeps on device */
+    int csc;                     /* also produce out-degree (for PageRank)    */
+    int threads;                 /* 0 = all cores                             */
+} asc_gen_params;
+void asc_gen_params_init(asc_gen_params* p);
+
+/* Host CSR owned by the library (free with asc_host_csr_free). */
+typedef struct {
+    uint64_t num_vertices;
+    uint64_t num_edges;
+    int kind;
+    uint64_t* off;               /* V+1 entries                               */
+    uint32_t* edges;             /* E ids, or 2E words for weighted AoS       */
+    uint32_t* out_degree;        /* V entries when kind == ASC_KIND_CSC       */
+} asc_host_csr;
+
+/* Generates a graph: self-loops and duplicate edges removed, adjacency sorted. */
+int asc_generate(const asc_gen_params* p, asc_host_csr* out);
+void asc_host_csr_free(asc_host_csr* h);
+/* View over a host CSR (for asc_graph_from_csr / asc_write_graph). */
+void asc_host_csr_view(const asc_host_csr* h, asc_csr_view* v);
+/* Builds a graph directly from a generated CSR, taking ownership of its arrays
+ * (no copy; *h is emptied). */
+int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph** out);
+
+/* ---- batched query serving --------------------------------------------- */
+/* No counterpart in the reference (which runs one traversal per process).
+ * A server keeps up to `lanes` (<= 64) BFS/SSSP queries in flight over one
+ * resident graph: each iteration of a multi-source kernel advances every
+ * admitted query by one frontier step; queries that finish free their lane
+ * and waiting queries are admitted at the next iteration boundary, subject
+ * to the device memory their state needs. */
+typedef enum { ASC_QUERY_BFS = 0, ASC_QUERY_SSSP = 1 } asc_query_kind;
+
+typedef struct {
+    int lanes;                   /* max queries in flight (1..64), 0 = 64     */
+    uint64_t state_budget_bytes; /* device bytes for per-query state; 0 = auto */
+    int queue_capacity;          /* max submitted-but-unfinished queries      */
+    int keep_results_on_device;  /* 1: results are fetched with asc_fetch     */
+} asc_server_opts;
+void asc_server_opts_init(asc_server_opts* o);
+
+typedef struct asc_server asc_server;
+
+typedef struct {
+    double submit_to_admit_ms;
+    double submit_to_done_ms;
+    uint32_t iterations;
+    uint32_t lane;
+} asc_qstats;
+
+int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** out);
+int asc_server_destroy(asc_server* s);
+/* Thread-safe.  Queues a query; *ticket identifies it. */
+int asc_submit(asc_server* s, int kind, uint32_t source, uint64_t* ticket);
+/* Thread-safe.  Blocks until the query finished; copies its V values into out
+ * (may be NULL to drop them) and releases the ticket. */
+int asc_wait(asc_server* s, uint64_t ticket, uint32_t* out, asc_qstats* qs);
+/* Runs the server loop until every submitted query finished (also runs in a
+ * background thread started by asc_server_create). */
+int asc_server_drain(asc_server* s);
+
+/* ---- autotuning -------------------------------------------------------- */
+typedef struct {
+    int heavy_threshold;
+    int pr_group;
+    int sssp_delta;
+    int serve_lanes;
+    double score_ms;             /* best measured time of the tuned workload  */
+} asc_tuning;
+/* Sweeps the tunables on sampled sources of g and stores the best values in
+ * the graph (and in *out).  `budget_ms` bounds the sweep's wall time. */
+int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out);
+int asc_tuning_save(const char* path, const asc_graph* g, const asc_tuning* t);
+int asc_tuning_load(const char* path, asc_graph* g, asc_tuning* t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASCETIC_H */

@@ -1,0 +1,162 @@
+"""GPU parity: BFS / SSSP / CC through the C ABI against the oracle.
+
+Integer results must be bit-exact.  Every case is checked with the graph
+fully resident and with most of its edges on demand (small device budget,
+small parts), in both streaming modes.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import ascetic_amd as A
+from ascetic_amd import _abi
+from oracle import oracle as O
+from tests import graphs as G
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_graph(ctx, off, dst, w=None, off32=False):
+    off = np.asarray(off, np.uint32 if off32 else np.uint64)
+    edges = dst if w is None else G.wedges(dst, w)
+    return ctx.from_arrays(off, edges)
+
+
+CASES = [
+    ("path", lambda: G.path_graph(50)),
+    ("star", lambda: G.star_graph(300)),
+    ("two_components", lambda: G.two_components(40, 25)),
+    ("random_sym", lambda: G.random_graph(5000, 20000, 1, symmetric=True)),
+    ("random_dir", lambda: G.random_graph(5000, 20000, 2, symmetric=False)),
+    ("hub", lambda: G.csr_from_edges(6000, [(0, i) for i in range(1, 6000)] +
+                                     [(i, i + 1) for i in range(1, 5999)], symmetric=True)),
+]
+
+
+def weights_for(off, seed):
+    return np.random.default_rng(seed).integers(1, 65, int(off[-1])).astype(np.uint32)
+
+
+@pytest.mark.parametrize("name,mk", CASES)
+@pytest.mark.parametrize("off32", [False, True])
+def test_bfs_sssp_cc_resident(gpu_ctx, name, mk, off32):
+    say("case", name, "off32", off32)
+    off, dst, _ = mk()
+    V = len(off) - 1
+    # symmetric weights so a symmetric graph stays symmetric
+    w = weights_for(off, 3)
+    csr = O.CSR(off, dst, w)
+    srcs = [0, int(np.argmax(np.diff(off))), V - 1]
+    with make_graph(gpu_ctx, off, dst, off32=off32) as g:
+        for s in srcs:
+            lv, st = g.bfs(s)
+            assert np.array_equal(lv, O.bfs_queue(csr, s)), (name, s)
+            assert st["reached"] == int((lv != V + 1).sum())
+        lab, _ = g.cc()
+        assert np.array_equal(lab, O.cc_minreach(csr)), name
+    with make_graph(gpu_ctx, off, dst, w, off32=off32) as g:
+        for s in srcs:
+            d, _ = g.sssp(s)
+            assert np.array_equal(d, O.dijkstra(csr, s)), (name, s)
+            lv, _ = g.bfs(s)
+            assert np.array_equal(lv, O.bfs_queue(csr, s))
+
+
+@pytest.mark.parametrize("mode", [_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY])
+@pytest.mark.parametrize("name,mk", CASES[3:])
+def test_streaming_matches_resident(name, mk, mode):
+    say("streaming", name, "mode", mode)
+    off, dst, _ = mk()
+    V = len(off) - 1
+    w = weights_for(off, 5)
+    csr = O.CSR(off, dst, w)
+    E = int(off[-1])
+    # static prefix ~ 1/3 of the edges, parts of 4096 edges (a hub spans parts)
+    with A.Context(edge_budget_bytes=4 * (E // 3), part_bytes=4 * 4096, stream_mode=mode) as ctx:
+        with make_graph(ctx, off, dst) as g:
+            info = g.info()
+            # (the hub case's first vertex already crosses the 4096-edge
+            # floored prefix, so nothing but isolated vertices is static)
+            assert info["static_vertices"] < V and info["ondemand_edges"] > 0
+            for s in (0, int(np.argmax(np.diff(off)))):
+                lv, st = g.bfs(s)
+                assert np.array_equal(lv, O.bfs_queue(csr, s))
+                assert st["streamed_edges"] > 0 or s == 0
+            lab, st = g.cc()
+            assert np.array_equal(lab, O.cc_minreach(csr))
+            assert st["streamed_edges"] > 0
+    with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_mode=mode) as ctx:
+        with make_graph(ctx, off, dst, w) as g:
+            for s in (0, int(np.argmax(np.diff(off)))):
+                d, _ = g.sssp(s)
+                assert np.array_equal(d, O.dijkstra(csr, s))
+
+
+def test_everything_on_demand(gpu_ctx):
+    off, dst, _ = G.random_graph(3000, 12000, 9, symmetric=False)
+    w = weights_for(off, 9)
+    csr = O.CSR(off, dst, w)
+    with A.Context(edge_budget_bytes=1, part_bytes=8 * 1024) as ctx:
+        with make_graph(ctx, off, dst, w) as g:
+            assert g.info()["static_vertices"] <= 1
+            d, st = g.sssp(5)
+            assert np.array_equal(d, O.dijkstra(csr, 5))
+            assert st["parts"] >= 1
+            lab, _ = g.cc()
+            assert np.array_equal(lab, O.cc_minreach(csr))
+
+
+def test_value_conventions():
+    off, dst, _ = G.two_components(5, 3)
+    with A.Context(value_convention=_abi.VALUES_ZERO_BASED) as ctx:
+        with make_graph(ctx, off, dst) as g:
+            lv, _ = g.bfs(1)
+    assert lv.tolist() == [1, 0, 1, 2, 3] + [0xFFFFFFFF] * 3
+
+
+def test_errors(gpu_ctx):
+    off, dst, _ = G.path_graph(10)
+    with make_graph(gpu_ctx, off, dst) as g:
+        with pytest.raises(A.AscError) as e:
+            g.bfs(10)
+        assert e.value.code == _abi.ERR_INVALID
+        with pytest.raises(A.AscError) as e:
+            g.sssp(0)
+        assert e.value.code == _abi.ERR_UNSUPPORTED
+
+
+def test_rmat_scale20_bfs_from_0(gpu_ctx):
+    """R-MAT scale 20, edge factor 16, symmetrized, u32 offsets; BFS from 0."""
+    say("rmat s20 bfs")
+    h = A.rmat(20, 16, seed=1)
+    off, dst = h.offsets().copy(), h.dst().copy()
+    ref = O.bfs_queue(O.CSR(off, dst), 0)
+    with gpu_ctx.adopt(h, offset_bytes=4) as g:
+        for _ in range(2):
+            lv, st = g.bfs(0)
+            assert np.array_equal(lv, ref)
+        assert st["reached"] == int((ref != len(off)).sum())
+
+
+def test_rmat_scale18_sssp_budget40(gpu_ctx):
+    """Weighted R-MAT (w in [1,64]), source = max-degree vertex, device edge
+    budget 40% of the edge+weight bytes: the rest streams each iteration."""
+    say("rmat s18 sssp budget 40%")
+    h = A.rmat(18, 16, seed=2, weighted=True, wmin=1, wmax=64)
+    off = h.offsets().copy()
+    ew = h.edge_words().copy()
+    E = int(off[-1])
+    src = int(np.argmax(np.diff(off)))
+    ref = O.dijkstra(O.CSR(off, ew[:, 0], ew[:, 1]), src)
+    for mode in (_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY):
+        with A.Context(edge_budget_bytes=int(0.4 * 8 * E), part_bytes=8 << 20, stream_mode=mode) as ctx:
+            with ctx.from_arrays(off, ew) as g:
+                d, st = g.sssp(src)
+                assert np.array_equal(d, ref)
+                assert st["streamed_edges"] > 0
