@@ -176,7 +176,7 @@ class Context:
 
     def __init__(self, device=0, advise_rate=0.0, edge_budget_bytes=0, part_bytes=0, device_mem_limit=0,
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
-                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False):
+                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -185,6 +185,7 @@ class Context:
         cfg.stream_mode, cfg.value_convention = stream_mode, value_convention
         cfg.heavy_threshold, cfg.pr_group, cfg.sssp_delta, cfg.verbose = (
             heavy_threshold, pr_group, sssp_delta, int(verbose))
+        cfg.profile = int(profile)
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")
         self._h = h
@@ -263,21 +264,20 @@ class Graph:
     def replan(self, edge_budget_bytes=0, advise_rate=0.0):
         check(_abi.load().asc_graph_replan(self._h, edge_budget_bytes, advise_rate), "asc_graph_replan")
 
-    def _run(self, fn, *args):
-        V = self.num_vertices
-        out = np.empty(V, dtype=np.uint32)
+    def _run(self, fn, *args, output=True):
+        out = np.empty(self.num_vertices, dtype=np.uint32) if output else None
         st = _abi.Stats()
-        check(fn(self._h, *args, _u32p(out), C.byref(st)), fn.__name__)
+        check(fn(self._h, *args, _u32p(out) if output else None, C.byref(st)), fn.__name__)
         return out, st.as_dict()
 
-    def bfs(self, source=0):
-        return self._run(_abi.load().asc_bfs, source)
+    def bfs(self, source=0, output=True):
+        return self._run(_abi.load().asc_bfs, source, output=output)
 
-    def sssp(self, source=0):
-        return self._run(_abi.load().asc_sssp, source)
+    def sssp(self, source=0, output=True):
+        return self._run(_abi.load().asc_sssp, source, output=output)
 
-    def cc(self):
-        return self._run(_abi.load().asc_cc)
+    def cc(self, output=True):
+        return self._run(_abi.load().asc_cc, output=output)
 
     def pagerank(self, mode=_abi.PR_NORMALISED, damping=0.85, iterations=20, threshold=0.001,
                  max_iterations=0):

@@ -48,6 +48,7 @@ class Config(C.Structure):
         ("pr_group", C.c_int),
         ("sssp_delta", C.c_int),
         ("verbose", C.c_int),
+        ("profile", C.c_int),
     ]
 
 
@@ -80,6 +81,19 @@ class GraphInfo(C.Structure):
     ]
 
 
+class KernelStats(C.Structure):
+    _fields_ = [
+        ("launches", C.c_uint32),
+        ("pad", C.c_uint32),
+        ("ms", C.c_double),
+        ("edges", C.c_uint64),
+        ("vertices", C.c_uint64),
+    ]
+
+
+KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull")
+
+
 class Stats(C.Structure):
     _fields_ = [
         ("total_ms", C.c_double),
@@ -101,10 +115,14 @@ class Stats(C.Structure):
         ("kernel_ms", C.c_double),
         ("pr_residual", C.c_double),
         ("host_cores", C.c_int),
+        ("kern", KernelStats * 4),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_}
+        d = {name: getattr(self, name) for name, _ in self._fields_ if name != "kern"}
+        d["kernels"] = {KERNEL_NAMES[i]: {n: getattr(self.kern[i], n) for n in ("launches", "ms", "edges", "vertices")}
+                        for i in range(4)}
+        return d
 
 
 class PrOpts(C.Structure):

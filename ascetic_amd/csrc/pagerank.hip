@@ -352,6 +352,10 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         st->reached_degree_sum = g->h.E;
         st->pr_residual = hres[iters - 1];
         st->host_cores = hardware_threads();
+        st->kern[3].launches = (uint32_t)iters;
+        st->kern[3].ms = kms;
+        st->kern[3].edges = g->h.E * (uint64_t)iters;
+        st->kern[3].vertices = V * (uint64_t)iters;
     }
     return ASC_OK;
 }

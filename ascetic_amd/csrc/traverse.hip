@@ -125,8 +125,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
     st_res.flush(a.q_out, &nxt->n_res);
     st_od.flush(a.od_out, &nxt->n_od);
     if (tid == 0 && my_verts) {
-        atomicAdd(&a.stats->edges_static, my_edges);
-        atomicAdd(&a.stats->verts, my_verts);
+        atomicAdd(&a.stats->edges_local, my_edges);
+        atomicAdd(&a.stats->verts_local, my_verts);
     }
 }
 
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
     }
     st_res.flush(a.q_out, &nxt->n_res);
     st_od.flush(a.od_out, &nxt->n_od);
-    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_static, my_edges);
+    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
 }
 
 // Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
     WaveStage<kStageCap> st_res{s_stage[0][w], 0};
     WaveStage<kStageCap> st_od{s_stage[1][w], 0};
-    unsigned long long my_edges = 0;
+    unsigned long long my_edges = 0, my_verts = 0;
 
     for (uint64_t t0 = p.lo + (uint64_t)blockIdx.x * kTile; t0 < p.hi; t0 += (uint64_t)gridDim.x * kTile) {
         const uint64_t t1 = min(t0 + (uint64_t)kTile, p.hi);
@@ -255,11 +255,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
             st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
         }
         my_edges += tlen;
+        my_verts += m;
         __syncthreads();
     }
     st_res.flush(a.q_out, &nxt->n_res);
     st_od.flush(a.od_out, &nxt->n_od);
-    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_part, my_edges);
+    if (tid == 0 && my_edges) {
+        atomicAdd(&a.stats->edges_part, my_edges);
+        atomicAdd(&a.stats->verts_part, my_verts);
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
@@ -370,6 +374,28 @@ static hipEvent_t pooled_event(TravState* t, size_t i) {
     return t->ev_pool[i];
 }
 
+static hipEvent_t next_event(TravState* t) { return pooled_event(t, t->ev_next++); }
+
+// Launches through `launch`; when profiling, brackets it with an event pair on
+// the launch stream and records the pair under `kind`.
+template <class F>
+static int timed_launch(TravState* t, bool prof, int kind, hipStream_t s, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof) {
+        a = next_event(t);
+        b = next_event(t);
+        if (!a || !b) return ASC_ERR_DEVICE;
+        ASC_HIP(hipEventRecord(a, s));
+    }
+    launch();
+    ASC_HIP(hipGetLastError());
+    if (prof) {
+        ASC_HIP(hipEventRecord(b, s));
+        t->timed.push_back({kind, a, b});
+    }
+    return ASC_OK;
+}
+
 int effective_heavy_threshold(const asc_graph* g) {
     uint64_t thr = g->heavy_threshold > 0 ? (uint64_t)g->heavy_threshold : 2048;
     const uint64_t floor_cap = (g->h.E + kHeavyCap - 1) / kHeavyCap;  // list can never overflow
@@ -406,6 +432,7 @@ static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, 
                         RunCounters& rc) {
     asc_ctx* ctx = g->ctx;
     TravState* t = g->trav;
+    const bool prof = ctx->cfg.profile != 0;
     const double t_begin = now_ms();
     // 1. on-demand list of this level (written by the previous level's kernels,
     //    which completed before this call)
@@ -464,9 +491,10 @@ static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, 
         p.zc = (const EdgeT*)dptr;
         p.lo = 0;
         p.hi = T;
-        hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, ctx->s_main, a,
-                           level, p);
-        ASC_HIP(hipGetLastError());
+        ASC_TRY(timed_launch(t, prof, KERN_PART, ctx->s_main, [&] {
+            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, ctx->s_main,
+                               a, level, p);
+        }));
         rc.parts += 1;
     } else {
         const uint64_t PE = t->part_edges;
@@ -485,9 +513,10 @@ static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, 
             p.part = (const EdgeT*)t->part[b];
             p.lo = lo;
             p.hi = hi;
-            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0, ctx->s_main,
-                               a, level, p);
-            ASC_HIP(hipGetLastError());
+            ASC_TRY(timed_launch(t, prof, KERN_PART, ctx->s_main, [&] {
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0,
+                                   ctx->s_main, a, level, p);
+            }));
             ASC_HIP(hipEventRecord(t->ev_kern[b], ctx->s_main));
         }
         rc.parts += (uint32_t)np;
@@ -549,7 +578,9 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         no = src < vs ? 0 : 1;
     }
     RunCounters rc{};
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> kev;
+    const bool prof = ctx->cfg.profile != 0;
+    t->timed.clear();
+    t->ev_next = 0;
     const double t0 = now_ms();
     uint32_t level = 0;
     for (;; level++) {
@@ -558,14 +589,24 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         a.q_in = t->q[level & 1];
         a.q_out = t->q[(level + 1) & 1];
         a.od_out = t->od[(level + 1) & 1];
-        hipEvent_t e0 = pooled_event(t, 2 * level), e1 = pooled_event(t, 2 * level + 1);
-        if (!e0 || !e1) return ASC_ERR_DEVICE;
-        ASC_HIP(hipEventRecord(e0, s));
-        hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
-        hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
-        ASC_HIP(hipGetLastError());
-        ASC_HIP(hipEventRecord(e1, s));
-        kev.emplace_back(e0, e1);
+        hipEvent_t e0 = nullptr;
+        if (!prof) {
+            e0 = next_event(t);
+            if (!e0) return ASC_ERR_DEVICE;
+            ASC_HIP(hipEventRecord(e0, s));
+        }
+        ASC_TRY(timed_launch(t, prof, KERN_LOCAL, s, [&] {
+            hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        }));
+        ASC_TRY(timed_launch(t, prof, KERN_HEAVY, s, [&] {
+            hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        }));
+        if (!prof) {
+            hipEvent_t e1 = next_event(t);
+            if (!e1) return ASC_ERR_DEVICE;
+            ASC_HIP(hipEventRecord(e1, s));
+            t->timed.push_back({-1, e0, e1});
+        }
         if (no > 0) ASC_TRY((stream_level<ALGO, OffT, EdgeT>(g, a, level, no, grid, rc)));
         ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + ((level + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
@@ -573,31 +614,47 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         no = t->h_slot[0].n_od;
     }
     const double t1 = now_ms();
-    double kms = 0;
-    for (auto& pr : kev) {
+    double static_ms = 0;
+    asc_kernel_stats kern[4];
+    std::memset(kern, 0, sizeof(kern));
+    for (auto& x : t->timed) {
         float ms = 0;
-        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) kms += ms;
+        if (hipEventElapsedTime(&ms, x.a, x.b) != hipSuccess) continue;
+        if (x.kind < 0) {
+            static_ms += ms;
+        } else {
+            kern[x.kind].ms += ms;
+            kern[x.kind].launches++;
+        }
     }
+    if (prof) static_ms = kern[KERN_LOCAL].ms + kern[KERN_HEAVY].ms;
     ASC_HIP(hipMemcpy(t->h_stats.p, t->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
+    const DevStats& ds = t->h_stats[0];
+    kern[KERN_LOCAL].edges = ds.edges_local;
+    kern[KERN_LOCAL].vertices = ds.verts_local;
+    kern[KERN_HEAVY].edges = ds.edges_heavy;
+    kern[KERN_PART].edges = ds.edges_part;
+    kern[KERN_PART].vertices = ds.verts_part;
     if (out) ASC_HIP(hipMemcpy(out, t->val, 4 * V, hipMemcpyDeviceToHost));
 
     if (st) {
         std::memset(st, 0, sizeof(*st));
         st->total_ms = t1 - t0;
         st->pre_ms = rc.pre_ms;
-        st->static_ms = kms;
-        st->kernel_ms = kms;
+        st->static_ms = static_ms;
+        st->kernel_ms = static_ms + kern[KERN_PART].ms;
         st->overload_ms = rc.overload_ms;
         st->fill_ms = rc.fill_ms;
         st->iterations = level;
         st->parts = rc.parts;
         st->node_sum = rc.node_sum;
-        st->static_edges_traversed = t->h_stats[0].edges_static;
-        st->edges_traversed = t->h_stats[0].edges_static + t->h_stats[0].edges_part;
+        st->static_edges_traversed = ds.edges_local + ds.edges_heavy;
+        st->edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_part;
         st->streamed_edges = rc.streamed_edges;
         st->streamed_bytes = rc.streamed_edges * g->h.edge_bytes();
-        st->vertices_processed = t->h_stats[0].verts;
+        st->vertices_processed = ds.verts_local;
         st->host_cores = hardware_threads();
+        std::memcpy(st->kern, kern, sizeof(kern));
     }
     if (out) {
         // reached vertices and their degree sum (Graph500 TEPS numerator)

@@ -26,11 +26,15 @@ struct Slot {
 };
 
 struct DevStats {
-    unsigned long long edges_static;
-    unsigned long long edges_part;
-    unsigned long long verts;
-    unsigned long long pad;
+    unsigned long long edges_local;   // k_expand_local
+    unsigned long long verts_local;
+    unsigned long long edges_heavy;   // k_expand_heavy
+    unsigned long long edges_part;    // k_expand_part
+    unsigned long long verts_part;
+    unsigned long long pad[3];
 };
+
+enum { KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3 };
 
 template <class OffT, class EdgeT>
 struct TravArgs {
@@ -91,6 +95,13 @@ struct TravState {
     HostBuf<char> stage[2];
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr}, ev_meta = nullptr;
     std::vector<hipEvent_t> ev_pool;
+    // event pairs recorded during the current run: (kernel kind, start, end)
+    struct Timed {
+        int kind;
+        hipEvent_t a, b;
+    };
+    std::vector<Timed> timed;
+    size_t ev_next = 0;
 };
 
 int traverse(asc_graph* g, int algo, uint32_t src, uint32_t* out, asc_stats* st);

@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark on one MI355X.
+
+Workload (default): SSSP on an R-MAT graph, scale 24 (2^24 vertices), edge
+factor 16, (a,b,c,d) = (0.57,0.19,0.19,0.05), symmetrized, self-loops and
+duplicates removed, u32 weights uniform in [1,64] drawn per unordered pair,
+u64 offsets and u32 ids (AoS {dst,w} edge records), source = highest-degree
+vertex.  The device edge budget is 40% of the edge+weight bytes, so the
+static/on-demand partition streams the other 60% from host memory every
+iteration it is needed.
+
+A step is one SSSP from the source over the whole graph (inputs resident in
+HBM / pinned host memory before timing starts; the result stays on the
+device).  `value` is GTEPS in the Graph500 convention: undirected edges in
+the traversed component (sum of reached out-degrees / 2) / seconds per step.
+
+The JSON line also carries:
+  roofline     the dominant relax kernel's algorithmic bytes / its summed
+               launch time (HIP events on its launch stream), vs 8 TB/s HBM;
+  link         streamed on-demand bytes / step time;
+  cpu_baseline the oracle's multi-threaded restatement of the reference
+               algorithm on the host cores, same graph and source (kind
+               "port"), whose result is also compared with the GPU's.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GTEPS per algo on 1 MI355X; batched BFS/SSSP queries/s + p99 latency"
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(kern, s_e, s_o):
+    """Bytes a work-optimal push relax must move: per edge the record and one
+    destination value; per frontier vertex an offset pair, its value, a queue
+    pop and a push (per list entry of a streamed part: id, u64 prefix, value)."""
+    return {
+        "expand_local": kern["expand_local"]["edges"] * (s_e + 4) + kern["expand_local"]["vertices"] * (2 * s_o + 12),
+        "expand_heavy": kern["expand_heavy"]["edges"] * (s_e + 4),
+        "expand_part": kern["expand_part"]["edges"] * (s_e + 4) + kern["expand_part"]["vertices"] * 16,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=24)
+    ap.add_argument("--budget-frac", type=float, default=0.4)
+    ap.add_argument("--part-mb", type=int, default=256)
+    ap.add_argument("--stream-mode", choices=["gather", "zerocopy"], default="gather")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one MI355X only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the traversal would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import ascetic_amd as A
+    from ascetic_amd import _abi
+
+    t = time.perf_counter()
+    h = A.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=True, wmin=1, wmax=64)
+    off = h.offsets()
+    ew = h.edge_words()
+    V, E = h.num_vertices, h.num_edges
+    deg = np.diff(off)
+    src = int(np.argmax(deg))
+    log(f"generated R-MAT s{args.scale} ef{args.edge_factor}: V={V} E={E} max-degree {int(deg[src])} at {src} "
+        f"({time.perf_counter() - t:.1f}s)")
+    s_e, s_o = 8, 8
+    budget = int(args.budget_frac * E * s_e)
+    mode = _abi.STREAM_GATHER if args.stream_mode == "gather" else _abi.STREAM_ZEROCOPY
+    ctx = A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, stream_mode=mode, profile=True)
+    g = ctx.from_arrays(off, ew)
+    info = g.info()
+    log(f"plan: static edges {info['static_edges']} ({info['static_edges'] / E:.1%}), static vertices "
+        f"{info['static_vertices']}, on-demand edges {info['ondemand_edges']}, part {info['part_edges']} edges")
+
+    # warm-up (the first run also returns the distances for the parity check)
+    dist, st0 = g.sssp(src)
+    for _ in range(max(args.warmup - 1, 0)):
+        g.sssp(src, output=False)
+    reached_deg = st0["reached_degree_sum"]
+
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    runs = [g.sssp(src, output=False)[1] for _ in range(args.steps)]
+    ctx.synchronize()
+    T = time.perf_counter() - t0
+    step_s = T / args.steps
+    gteps = (reached_deg / 2) / step_s / 1e9
+
+    # per-kernel live timing and algorithmic bytes over the timed steps
+    kern = {}
+    for name in ("expand_local", "expand_heavy", "expand_part"):
+        kern[name] = {k: sum(r["kernels"][name][k] for r in runs) for k in ("launches", "ms", "edges", "vertices")}
+    abytes = algorithmic_bytes(kern, s_e, s_o)
+    dom = max(abytes, key=lambda k: kern[k]["ms"])
+    dk = kern[dom]
+    achieved = abytes[dom] / (dk["ms"] / 1e3) / 1e9 if dk["ms"] > 0 else 0.0
+    streamed = sum(r["streamed_bytes"] for r in runs) / args.steps
+
+    result = {
+        "metric": METRIC,
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"sssp_rmat_s{args.scale}_ef{args.edge_factor}_w1-64_budget{int(args.budget_frac * 100)}",
+            "graph": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, (0.57,0.19,0.19,0.05), "
+                     f"symmetrized, seed {args.seed}",
+            "V": V, "E_directed": E, "source": src, "offsets": "u64", "edge_record": "{u32 dst, u32 w}",
+            "edge_budget_bytes": budget, "static_edges": info["static_edges"],
+            "ondemand_edges": info["ondemand_edges"], "part_bytes": args.part_mb << 20,
+            "stream_mode": args.stream_mode,
+            "iterations": st0["iterations"], "reached": st0["reached"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom,
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBPS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None,
+            "launches_per_step": dk["launches"] / args.steps,
+            "algorithmic_bytes_per_launch": abytes[dom] / max(dk["launches"], 1),
+            "avg_launch_us": dk["ms"] * 1e3 / max(dk["launches"], 1),
+            "kernels": {k: {"ms_per_step": kern[k]["ms"] / args.steps, "launches_per_step": kern[k]["launches"] /
+                            args.steps, "GBps": (abytes[k] / (kern[k]["ms"] / 1e3) / 1e9) if kern[k]["ms"] else 0}
+                        for k in kern},
+        },
+        "link": {
+            "streamed_bytes_per_step": streamed,
+            "achieved_GBps": round(streamed / step_s / 1e9, 3),
+            "gather_ms_per_step": sum(r["fill_ms"] for r in runs) / args.steps,
+            "overload_ms_per_step": sum(r["overload_ms"] for r in runs) / args.steps,
+        },
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+        nt = args.cpu_threads or min(16, os.cpu_count() or 1)
+        csr = O.CSR(off, ew[:, 0], ew[:, 1])
+        tc = time.perf_counter()
+        ref, cst = O.sssp_ref(csr, src, info["static_edges"], info["part_edges"], nt)
+        tcpu = time.perf_counter() - tc
+        ok = bool(np.array_equal(ref, dist))
+        result["cpu_baseline"] = {
+            "value": round((reached_deg / 2) / tcpu / 1e9, 4),
+            "unit": "GTEPS",
+            "cores": nt,
+            "kind": "port",
+            "sample": f"one full SSSP of the same graph and source by the oracle's restatement of "
+                      f"CalculateOpt.cu sssp_opt (same static prefix and part size), {tcpu:.2f} s",
+        }
+        result["parity"] = "bit-exact vs oracle" if ok else "MISMATCH vs oracle"
+        if not ok:
+            log("GPU distances differ from the oracle")
+    print(json.dumps(result), flush=True)
+    g.close()
+    ctx.close()
+    if result.get("parity", "").startswith("MISMATCH"):
+        sys.exit(1)
+
+
+if __name__ == "__main__":
+    main()

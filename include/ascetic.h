@@ -112,6 +112,8 @@ typedef struct {
     int sssp_delta;              /* SSSP bucket width (near-far); 0 = tuned,
                                     <0 = plain frontier Bellman-Ford          */
     int verbose;                 /* print per-phase timings like the reference */
+    int profile;                 /* 1: time every relax kernel with HIP events
+                                    on its launch stream (asc_stats.kern)    */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -172,6 +174,14 @@ int asc_graph_replan(asc_graph* g, uint64_t edge_budget_bytes, double advise_rat
 
 /* ---- per-run statistics (phase names follow CalculateOpt.cu:30-36) ------ */
 typedef struct {
+    uint32_t launches;
+    uint32_t pad;
+    double ms;                   /* summed launch durations (HIP events)      */
+    uint64_t edges;              /* edge records relaxed                      */
+    uint64_t vertices;           /* frontier / list entries expanded          */
+} asc_kernel_stats;
+
+typedef struct {
     double total_ms;             /* first iteration start .. last end         */
     double pre_ms;               /* host-side frontier bookkeeping            */
     double static_ms;            /* resident relax kernels (HIP events)       */
@@ -191,6 +201,10 @@ typedef struct {
     double kernel_ms;            /* sum of relax-kernel durations (events)    */
     double pr_residual;          /* PageRank: last iteration's L1 residual    */
     int host_cores;
+    /* Per relax kernel, filled when asc_config.profile = 1 (else zero):
+     * [0] resident frontier expansion, [1] grid-cooperative heavy vertices,
+     * [2] on-demand parts, [3] PageRank pull. */
+    asc_kernel_stats kern[4];
 } asc_stats;
 
 /* ---- algorithms (CalculateOpt.cu:14-721) ------------------------------- */
