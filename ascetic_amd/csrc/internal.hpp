@@ -141,6 +141,11 @@ int hardware_threads();
 // Process-wide pool for host-side preprocessing (generation, loading).
 ThreadPool& host_pool();
 
+// Copies edges [lo, hi) of the concatenated adjacency of list[0..m)
+// (list[k] owns [prefix[k], prefix[k+1])) from the host edge array into dst.
+void gather_edges(ThreadPool& pool, const char* edges, const uint64_t* off, size_t EB, const uint32_t* list,
+                  const unsigned long long* prefix, uint64_t m, uint64_t lo, uint64_t hi, char* dst);
+
 }  // namespace asc
 
 // ---- opaque handles ---------------------------------------------------------
@@ -179,6 +184,7 @@ struct asc_graph {
     int heavy_threshold = 0;
     int pr_group = 0;
     int sssp_delta = 0;
+    double mean_weight = 0;         // SSSP: mean edge weight (0 = not computed)
     int serve_lanes = 64;
     uint32_t max_degree = 0, max_degree_vertex = 0;
     asc::TravState* trav = nullptr;

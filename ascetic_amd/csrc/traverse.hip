@@ -4,22 +4,33 @@
 // relax kernels gpu_kernels.cuh:345-532 and the frontier bookkeeping
 // gpu_kernels.cu:111-122, gpu_kernels.cuh:399-438 plus Thrust scans.
 //
-// Per iteration ("level") the reference classifies the active flags into a
-// static and an overload list with several O(V) passes and host syncs.  Here
-// the relax kernels themselves append improved vertices to two device queues
-// (resident: v < static_vertices, on-demand: the rest) through per-wave LDS
-// staging, so a level costs O(frontier edges) and one host sync:
+// The reference classifies the active flags into a static and an overload
+// list with several O(V) passes and host syncs per iteration, and ships the
+// overload adjacency of every active vertex every iteration.  Here the relax
+// kernels append improved vertices to device queues through per-wave LDS
+// staging, so a step costs O(frontier edges) and one host sync:
 //   k_expand_local  one workgroup per 256 frontier vertices; degrees below the
 //                   heavy threshold are expanded edge-balanced inside the
 //                   workgroup (LDS prefix scan + per-lane binary search), so
 //                   consecutive lanes read consecutive edges of a vertex;
-//                   heavier vertices are appended to a grid-wide list.
+//                   heavier vertices go to a grid-wide list.
 //   k_expand_heavy  grid-cooperative, edge-balanced over the heavy list.
 //   k_expand_part   edge-balanced over one streamed part of on-demand edges
 //                   (or over mapped pinned host memory in zero-copy mode).
-// Relaxation is check-then-atomicMin (gpu_kernels.cuh:390-392); a vertex is
-// queued only when its atomic returned a larger value, and once per level
-// (BFS: visited bitmap; SSSP/CC: per-vertex level stamp).
+// Improved vertices are routed to one of three queues:
+//   resident frontier  v < static_vertices, expanded next step on the device;
+//   on-demand pile     other vertices, deduplicated, streamed in one batch;
+//   far pile           SSSP only: tentative distance >= theta (the near/far
+//                      band boundary), revisited when theta advances.
+// BFS stays level-synchronous: each level expands the resident frontier and
+// streams the pile filled by the previous level, so every reached on-demand
+// vertex crosses the link once.  SSSP and CC relax the resident frontier to
+// exhaustion before streaming the pile, and SSSP advances theta by a bucket
+// width only when both are empty, so on-demand vertices are mostly streamed
+// once their value is final.  A vertex already expanded at its current value
+// (`done`) is skipped.  Final values are unique, so the schedule does not
+// change results (the reference's own schedule is chaotic, CalculateOpt.cu:
+// 481-523).
 #include <algorithm>
 
 #include "device_common.hpp"
@@ -32,27 +43,56 @@ namespace asc {
 // device code
 // ---------------------------------------------------------------------------
 
+enum { TO_NONE = 0, TO_RES = 1, TO_PEND = 2, TO_FAR = 3 };
+
 template <int ALGO>
 __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
     return ALGO == ALG_SSSP ? sv + w : sv;  // BFS passes level + 2 as sv
 }
 
+// Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
+// and routing of an improved vertex.
 template <int ALGO, class OffT, class EdgeT>
-__device__ __forceinline__ bool relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
+__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
     if (ALGO == ALG_BFS) {
         const uint32_t bit = 1u << (v & 31);
         uint32_t* word = a.visited + (v >> 5);
-        if (*word & bit) return false;
-        if (atomicOr(word, bit) & bit) return false;
+        if (*word & bit) return TO_NONE;
+        if (atomicOr(word, bit) & bit) return TO_NONE;
         a.val[v] = cand;
-        return true;
+        return v < a.vs ? TO_RES : TO_PEND;
     } else {
-        if (cand >= a.val[v]) return false;
+        if (cand >= a.val[v]) return TO_NONE;
         const uint32_t old = atomicMin(a.val + v, cand);
-        if (old <= cand) return false;
-        return atomicMax(a.stamp + v, level + 1) < level + 1;
+        if (old <= cand) return TO_NONE;
+        if (ALGO == ALG_SSSP && cand >= a.theta)
+            return atomicMax(a.fstamp + v, a.fepoch) < a.fepoch ? TO_FAR : TO_NONE;
+        if (v < a.vs) return atomicMax(a.stamp + v, level + 1) < level + 1 ? TO_RES : TO_NONE;
+        return atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
     }
 }
+
+template <int ALGO>
+struct Pushes {
+    WaveStage<kStageCap> res, pend, far;
+
+    template <class A>
+    __device__ __forceinline__ void push(const A& a, Slot* nxt, int to, uint32_t v) {
+        res.push(to == TO_RES, v, a.q_out, &nxt->n_res);
+        pend.push(to == TO_PEND, v, a.pend_out, a.pend_count);
+        if (ALGO == ALG_SSSP) far.push(to == TO_FAR, v, a.far_out, a.far_count);
+    }
+    template <class A>
+    __device__ __forceinline__ void flush(const A& a, Slot* nxt) {
+        res.flush(a.q_out, &nxt->n_res);
+        pend.flush(a.pend_out, a.pend_count);
+        if (ALGO == ALG_SSSP) far.flush(a.far_out, a.far_count);
+    }
+};
+
+#define ASC_STAGE_DECL                                              \
+    __shared__ uint32_t s_stage[3][kWavesPerBlock][kStageCap];      \
+    Pushes<ALGO> pq{{s_stage[0][wave_id()], 0}, {s_stage[1][wave_id()], 0}, {s_stage[2][wave_id()], 0}}
 
 template <int ALGO, class OffT, class EdgeT>
 __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
@@ -60,20 +100,17 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
     __shared__ OffT s_beg[kBlock];
     __shared__ uint32_t s_sv[kBlock];
     __shared__ uint32_t s_tmp[kWavesPerBlock];
-    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
+    ASC_STAGE_DECL;
 
     Slot* cur = a.slots + (level & 3);
     Slot* nxt = a.slots + ((level + 1) & 3);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         Slot* z = a.slots + ((level + 2) & 3);
         z->n_res = 0;
-        z->n_od = 0;
         z->heavy = 0;
     }
     const uint32_t n = cur->n_res;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
-    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
     unsigned long long my_edges = 0, my_verts = 0;
 
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -83,8 +120,14 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
         if (i < n) {
             const uint32_t u = a.q_in[i];
             beg = a.off[u];
-            const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
-            sv = (ALGO == ALG_BFS) ? level + 2 : a.val[u];
+            uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+            if (ALGO == ALG_BFS) {
+                sv = level + 2;
+            } else {
+                sv = a.val[u];
+                if (a.done[u] == sv) d = 0;  // already expanded at this value
+                else a.done[u] = sv;
+            }
             if (d >= a.heavy_thr) {
                 const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
                 const uint32_t idx = (uint32_t)(p >> kHeavyShift);
@@ -104,7 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
         const uint32_t cnt = min(n - base, (uint32_t)kBlock);
         for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
             const uint32_t e = eb + lane;
-            bool push = false;
+            int to = TO_NONE;
             uint32_t v = 0;
             if (e < total) {
                 uint32_t k = 0;
@@ -113,17 +156,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
                     if (k + step < cnt && s_pref[k + step] <= e) k += step;
                 const EdgeT ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
                 v = edge_dst(ed);
-                push = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
             }
-            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
-            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+            pq.push(a, nxt, to, v);
         }
         my_edges += total;
         my_verts += cnt;
         __syncthreads();
     }
-    st_res.flush(a.q_out, &nxt->n_res);
-    st_od.flush(a.od_out, &nxt->n_od);
+    pq.flush(a, nxt);
     if (tid == 0 && my_verts) {
         atomicAdd(&a.stats->edges_local, my_edges);
         atomicAdd(&a.stats->verts_local, my_verts);
@@ -133,11 +174,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
 template <int ALGO, class OffT, class EdgeT>
 __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
     constexpr int kWin = 8;  // a tile spans <= kTile / kMinHeavy + 1 heavy entries
-    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
     __shared__ unsigned long long s_hb[kWin];
     __shared__ OffT s_hoff[kWin];
     __shared__ uint32_t s_hsv[kWin];
     __shared__ uint32_t s_k0;
+    ASC_STAGE_DECL;
 
     Slot* cur = a.slots + (level & 3);
     Slot* nxt = a.slots + ((level + 1) & 3);
@@ -146,8 +187,6 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
     const uint64_t total = h & kHeavyMask;
     if (nh == 0) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
-    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
     unsigned long long my_edges = 0;
 
     for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
@@ -171,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
         const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
         for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
             const uint32_t rr = r + lane;
-            bool push = false;
+            int to = TO_NONE;
             uint32_t v = 0;
             if (rr < tlen) {
                 const uint64_t e = t0 + rr;
@@ -181,36 +220,33 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
                     if (j < nwin && s_hb[j] <= e) k = j;
                 const EdgeT ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
                 v = edge_dst(ed);
-                push = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
             }
-            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
-            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+            pq.push(a, nxt, to, v);
         }
         my_edges += tlen;
         __syncthreads();
     }
-    st_res.flush(a.q_out, &nxt->n_res);
-    st_od.flush(a.od_out, &nxt->n_od);
+    pq.flush(a, nxt);
     if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
 }
 
 // Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
-// the on-demand list (list[k] owns [prefix[k], prefix[k+1]); no zero-degree
-// entries).  ZC = false: edges come from the device part buffer holding
-// [lo, hi).  ZC = true: from mapped pinned host memory (tail = edges from
-// static_edges on).
+// the streamed list (list[k] owns [prefix[k], prefix[k+1]), no zero-degree
+// entries; sv[k] is the value list[k] is expanded with).  ZC = false: edges
+// come from the device part buffer holding [lo, hi).  ZC = true: from mapped
+// pinned host memory (tail = edges from static_edges on).
 template <int ALGO, class OffT, class EdgeT, bool ZC>
-__global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p) {
+__global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p,
+                                                         const uint32_t* list_sv) {
     __shared__ unsigned long long s_p[kTile];
     __shared__ uint32_t s_sv[kTile];
     __shared__ unsigned long long s_src[ZC ? kTile : 1];
-    __shared__ uint32_t s_stage[2][kWavesPerBlock][kStageCap];
     __shared__ uint32_t s_k[2];
+    ASC_STAGE_DECL;
 
     Slot* nxt = a.slots + ((level + 1) & 3);
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    WaveStage<kStageCap> st_res{s_stage[0][w], 0};
-    WaveStage<kStageCap> st_od{s_stage[1][w], 0};
     unsigned long long my_edges = 0, my_verts = 0;
 
     for (uint64_t t0 = p.lo + (uint64_t)blockIdx.x * kTile; t0 < p.hi; t0 += (uint64_t)gridDim.x * kTile) {
@@ -228,16 +264,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
         const uint32_t k0 = s_k[0];
         const uint32_t m = s_k[1] - k0 + 1;
         for (uint32_t j = tid; j < m; j += kBlock) {
-            const uint32_t u = p.list[k0 + j];
             s_p[j] = p.prefix[k0 + j];
-            s_sv[j] = (ALGO == ALG_BFS) ? level + 2 : a.val[u];
-            if (ZC) s_src[j] = (unsigned long long)a.off[u] - p.tail_base;
+            s_sv[j] = (ALGO == ALG_BFS) ? level + 2 : list_sv[k0 + j];
+            if (ZC) s_src[j] = (unsigned long long)a.off[p.list[k0 + j]] - p.tail_base;
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)(t1 - t0);
         for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
             const uint32_t rr = r + lane;
-            bool push = false;
+            int to = TO_NONE;
             uint32_t v = 0;
             if (rr < tlen) {
                 const uint64_t e = t0 + rr;
@@ -249,51 +284,130 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
                 if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
                 else ed = p.part[e - p.lo];
                 v = edge_dst(ed);
-                push = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
             }
-            st_res.push(push && v < a.vs, v, a.q_out, &nxt->n_res);
-            st_od.push(push && v >= a.vs, v, a.od_out, &nxt->n_od);
+            pq.push(a, nxt, to, v);
         }
         my_edges += tlen;
         my_verts += m;
         __syncthreads();
     }
-    st_res.flush(a.q_out, &nxt->n_res);
-    st_od.flush(a.od_out, &nxt->n_od);
+    pq.flush(a, nxt);
     if (tid == 0 && my_edges) {
         atomicAdd(&a.stats->edges_part, my_edges);
         atomicAdd(&a.stats->verts_part, my_verts);
     }
 }
 
+// Compacts a pile before it is streamed: drops zero-degree vertices and (SSSP,
+// CC) vertices already expanded at their current value, and snapshots the
+// value each kept vertex will be expanded with.
+template <int ALGO, class OffT>
+__global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const uint32_t* pile, uint32_t n,
+                                                        uint32_t* val, uint32_t* done, uint32_t* out,
+                                                        uint32_t* out_sv, uint32_t* count) {
+    const uint32_t lane = lane_id();
+    for (uint32_t base = (blockIdx.x * kBlock) & ~(kWave - 1u); base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        uint32_t v = 0, d = 0;
+        if (i < n) {
+            v = pile[i];
+            keep = off[v + 1] > off[v];
+            if (ALGO != ALG_BFS && keep) {
+                d = val[v];
+                keep = done[v] != d;
+                if (keep) done[v] = d;
+            }
+        }
+        const uint64_t m = wave_ballot(keep);
+        if (m == 0) continue;
+        uint32_t at = 0;
+        if (lane == 0) at = atomicAdd(count, (uint32_t)__popcll(m));
+        at = __shfl(at, 0, kWave);
+        if (keep) {
+            const uint32_t pos = at + lanes_below(m);
+            out[pos] = v;
+            if (ALGO != ALG_BFS) out_sv[pos] = d;
+        }
+    }
+}
+
+// Moves the far pile's entries whose value fell below theta into the current
+// frontier (resident) or the on-demand pile; the rest stays far.
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_split_far(TravArgs<OffT, EdgeT> a, const uint32_t* far_in, uint32_t n,
+                                                      uint32_t level, uint32_t* q_level, Piles* piles) {
+    constexpr int ALGO = ALG_SSSP;
+    ASC_STAGE_DECL;
+    Slot* slot = a.slots + (level & 3);
+    uint32_t lmin = 0xFFFFFFFFu;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + threadIdx.x;
+        int to = TO_NONE;
+        uint32_t v = 0;
+        if (i < n) {
+            v = far_in[i];
+            const uint32_t d = a.val[v];
+            if (a.done[v] != d) {
+                if (d < a.theta) {
+                    if (v < a.vs) to = atomicMax(a.stamp + v, level) < level ? TO_RES : TO_NONE;
+                    else to = atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
+                } else if (atomicMax(a.fstamp + v, a.fepoch) < a.fepoch) {
+                    to = TO_FAR;
+                    lmin = min(lmin, d);
+                }
+            }
+        }
+        pq.res.push(to == TO_RES, v, q_level, &slot->n_res);
+        pq.pend.push(to == TO_PEND, v, a.pend_out, a.pend_count);
+        pq.far.push(to == TO_FAR, v, a.far_out, a.far_count);
+    }
+    pq.res.flush(q_level, &slot->n_res);
+    pq.pend.flush(a.pend_out, a.pend_count);
+    pq.far.flush(a.far_out, a.far_count);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) lmin = min(lmin, (uint32_t)__shfl_xor(lmin, d, kWave));
+    if (lane_id() == 0 && lmin != 0xFFFFFFFFu) atomicMin(&piles->min_far, lmin);
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
 
-// Level-0 frontier for BFS / SSSP: the source alone (GraphMeta.cuh:432-446).
-__global__ void k_init_source(uint32_t* val, uint32_t* visited, uint32_t* q, uint32_t* od, Slot* slots, uint32_t src,
-                              uint32_t vs) {
+// Step-0 frontier for BFS / SSSP: the source alone (GraphMeta.cuh:432-446).
+__global__ void k_init_source(uint32_t* val, uint32_t* visited, uint32_t* pstamp, uint32_t* q, uint32_t* pend,
+                              Slot* slots, Piles* piles, uint32_t src, uint32_t vs) {
     val[src] = 1;
     if (visited) visited[src >> 5] |= 1u << (src & 31);
     if (src < vs) {
         q[0] = src;
         slots[0].n_res = 1;
     } else {
-        od[0] = src;
-        slots[0].n_od = 1;
+        pend[0] = src;
+        piles->n_pend[0] = 1;
+        if (pstamp) pstamp[src] = 1;
     }
 }
 
 // CC: every vertex starts with its own id and active (GraphMeta.cuh:448-452).
-__global__ void k_init_cc(uint32_t* val, uint32_t* q, uint32_t* od, Slot* slots, uint32_t V, uint32_t vs) {
+__global__ void k_init_cc(uint32_t* val, uint32_t* pstamp, uint32_t* q, uint32_t* pend, Slot* slots, Piles* piles,
+                          uint32_t V, uint32_t vs) {
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
         val[v] = v;
-        if (v < vs) q[v] = v; else od[v - vs] = v;
+        if (v < vs) {
+            q[v] = v;
+        } else {
+            pend[v - vs] = v;
+            pstamp[v] = 1;
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         slots[0].n_res = vs;
-        slots[0].n_od = V - vs;
+        piles->n_pend[0] = V - vs;
     }
 }
 
@@ -306,11 +420,12 @@ void trav_free(TravState* t) {
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    f(t->val); f(t->stamp); f(t->visited);
-    f(t->q[0]); f(t->q[1]); f(t->od[0]); f(t->od[1]);
+    f(t->val); f(t->stamp); f(t->pstamp); f(t->fstamp); f(t->done); f(t->visited);
+    for (int i = 0; i < 2; i++) {
+        f(t->q[i]); f(t->pend[i]); f(t->far[i]); f(t->part[i]);
+    }
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
-    f(t->od_list); f(t->od_prefix); f(t->slots); f(t->stats);
-    f(t->part[0]); f(t->part[1]);
+    f(t->od_list); f(t->od_prefix); f(t->slots); f(t->piles); f(t->stats);
     for (int i = 0; i < 2; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
         if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
@@ -325,32 +440,34 @@ static int trav_ensure(asc_graph* g) {
     const size_t EB = g->h.edge_bytes();
     TravState* t = g->trav;
     const bool need_parts = g->static_vertices < V;
-    if (t && t->V == V && (!need_parts || (t->part[0] && t->part_edges >= g->part_edges))) return ASC_OK;
+    if (t && t->V == V && (!need_parts || (t->od_list && t->part_edges >= g->part_edges))) return ASC_OK;
     if (t) trav_free(t);
     t = new TravState();
     g->trav = t;
     t->V = V;
     ASC_HIP(hipSetDevice(g->ctx->device));
-    ASC_HIP(hipMalloc(&t->val, 4 * V));
-    ASC_HIP(hipMalloc(&t->stamp, 4 * V));
+    for (uint32_t** p : {&t->val, &t->stamp, &t->pstamp, &t->fstamp, &t->done, &t->q[0], &t->q[1], &t->far[0],
+                         &t->far[1]})
+        ASC_HIP(hipMalloc(p, 4 * V));
     ASC_HIP(hipMalloc(&t->visited, 4 * ((V + 31) / 32)));
-    for (int i = 0; i < 2; i++) {
-        ASC_HIP(hipMalloc(&t->q[i], 4 * V));
-        ASC_HIP(hipMalloc(&t->od[i], 4 * V));
-    }
     ASC_HIP(hipMalloc(&t->heavy_u, 4 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&t->heavy_sv, 4 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&t->heavy_base, 8 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&t->slots, sizeof(Slot) * 4));
+    ASC_HIP(hipMalloc(&t->piles, sizeof(Piles)));
     ASC_HIP(hipMalloc(&t->stats, sizeof(DevStats)));
     ASC_TRY(t->h_slot.alloc(4, true));
+    ASC_TRY(t->h_piles.alloc(1, true));
     ASC_TRY(t->h_stats.alloc(1, true));
     if (need_parts) {
-        ASC_HIP(hipMalloc(&t->od_list, 4 * V));
-        ASC_HIP(hipMalloc(&t->od_prefix, 8 * (V + 1)));
-        ASC_TRY(t->h_od.alloc(V, true));
-        ASC_TRY(t->h_list.alloc(V, true));
-        ASC_TRY(t->h_prefix.alloc(V + 1, true));
+        const uint64_t nod = V - g->static_vertices;
+        ASC_HIP(hipMalloc(&t->pend[0], 4 * nod));
+        ASC_HIP(hipMalloc(&t->pend[1], 4 * nod));
+        // od_list holds the filtered list and, after it, the snapshot values
+        ASC_HIP(hipMalloc(&t->od_list, 8 * nod));
+        ASC_HIP(hipMalloc(&t->od_prefix, 8 * (nod + 1)));
+        ASC_TRY(t->h_list.alloc(nod, true));
+        ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
         for (int i = 0; i < 2; i++) {
             if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER) {
@@ -405,84 +522,87 @@ int effective_heavy_threshold(const asc_graph* g) {
     return (int)thr;
 }
 
+// SSSP bucket width: configured, else twice the mean edge weight (computed
+// once per graph); a negative setting disables the far pile.
+static uint32_t sssp_delta(asc_graph* g) {
+    if (g->sssp_delta > 0) return (uint32_t)g->sssp_delta;
+    if (g->sssp_delta < 0) return 0;
+    if (g->mean_weight <= 0) {
+        const uint64_t E = g->h.E;
+        std::atomic<uint64_t> sum{0};
+        host_pool().parallel_for(E, [&](uint64_t b, uint64_t e) {
+            uint64_t s = 0;
+            for (uint64_t i = b; i < e; i++) s += g->h.edges[2 * i + 1];
+            sum += s;
+        });
+        g->mean_weight = E ? (double)sum.load() / (double)E : 1.0;
+    }
+    return std::max<uint32_t>(1, (uint32_t)(2.0 * g->mean_weight));
+}
+
 // Copies the adjacency of list entries overlapping [lo, hi) of the
 // concatenated on-demand edge list into `dst` (the reference's
 // fillEdgeArrByMultiThread, GraphMeta.cuh:469-514, cut to one part).
 static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long long* prefix, uint64_t m,
                         uint64_t lo, uint64_t hi, char* dst) {
-    const size_t EB = g->h.edge_bytes();
-    const char* src = (const char*)g->h.edges.p;
-    const uint64_t* off = g->h.off.p;
-    g->ctx->pool->parallel_for(hi - lo, [&](uint64_t b, uint64_t e) {
-        uint64_t a = lo + b;
-        const uint64_t end = lo + e;
-        uint64_t k = (uint64_t)(std::upper_bound(prefix, prefix + m, (unsigned long long)a) - prefix) - 1;
-        while (a < end) {
-            const uint64_t kend = std::min<uint64_t>(prefix[k + 1], end);
-            const uint64_t from = off[list[k]] + (a - prefix[k]);
-            std::memcpy(dst + (a - lo) * EB, src + from * EB, (kend - a) * EB);
-            a = kend;
-            k++;
-        }
-    });
+    gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, g->h.off.p, g->h.edge_bytes(), list, prefix, m, lo, hi,
+                 dst);
 }
 
+// Streams one pile: filter on the device, list to the host, u64 prefix of
+// degrees, then parts (gather + double-buffered copies) or zero-copy reads.
 template <int ALGO, class OffT, class EdgeT>
-static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, uint32_t no, int grid,
-                        RunCounters& rc) {
+static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, const uint32_t* pile, uint32_t npile,
+                       int grid, RunCounters& rc) {
     asc_ctx* ctx = g->ctx;
     TravState* t = g->trav;
     const bool prof = ctx->cfg.profile != 0;
+    hipStream_t s = ctx->s_main;
     const double t_begin = now_ms();
-    // 1. on-demand list of this level (written by the previous level's kernels,
-    //    which completed before this call)
-    ASC_HIP(hipMemcpyAsync(t->h_od.p, t->od[level & 1], 4 * (size_t)no, hipMemcpyDeviceToHost, ctx->s_copy));
-    ASC_HIP(hipStreamSynchronize(ctx->s_copy));
-    // 2. drop zero-degree entries, u64 prefix of degrees
+    const uint64_t nod = g->h.V - g->static_vertices;
+    uint32_t* list_sv = t->od_list + nod;
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
+    hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096)),
+                       dim3(kBlock), 0, s, a.off, pile, npile, a.val, a.done, t->od_list, list_sv, &t->piles->n_list);
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipStreamSynchronize(s));
+    const uint32_t m = t->h_piles[0].n_list;
+    if (m == 0) {
+        rc.pre_ms += now_ms() - t_begin;
+        return ASC_OK;
+    }
+    ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipStreamSynchronize(s));
+    // u64 prefix of the degrees, in parallel
     const uint64_t* off = g->h.off.p;
     const int nt = ctx->pool->size();
-    std::vector<uint64_t> cnt(nt + 1, 0), esum(nt + 1, 0);
-    const uint64_t chunk = (no + nt - 1) / nt;
+    std::vector<uint64_t> esum(nt + 1, 0);
+    const uint64_t chunk = (m + nt - 1) / nt;
     ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(no, tt * chunk), e = std::min<uint64_t>(no, b + chunk), c = 0, s = 0;
-        for (uint64_t i = b; i < e; i++) {
-            uint64_t d = off[t->h_od[i] + 1] - off[t->h_od[i]];
-            c += d > 0;
-            s += d;
-        }
-        cnt[tt + 1] = c;
-        esum[tt + 1] = s;
+        uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = 0;
+        for (uint64_t i = b; i < e; i++) acc += off[t->h_list[i] + 1] - off[t->h_list[i]];
+        esum[tt + 1] = acc;
     });
-    for (int i = 0; i < nt; i++) {
-        cnt[i + 1] += cnt[i];
-        esum[i + 1] += esum[i];
-    }
+    for (int i = 0; i < nt; i++) esum[i + 1] += esum[i];
     ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(no, tt * chunk), e = std::min<uint64_t>(no, b + chunk);
-        uint64_t c = cnt[tt], s = esum[tt];
+        uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = esum[tt];
         for (uint64_t i = b; i < e; i++) {
-            const uint32_t u = t->h_od[i];
-            uint64_t d = off[u + 1] - off[u];
-            if (!d) continue;
-            t->h_list[c] = u;
-            t->h_prefix[c] = s;
-            c++;
-            s += d;
+            t->h_prefix[i] = acc;
+            acc += off[t->h_list[i] + 1] - off[t->h_list[i]];
         }
     });
-    const uint64_t m = cnt[nt], T = esum[nt];
+    const uint64_t T = esum[nt];
     t->h_prefix[m] = T;
     rc.pre_ms += now_ms() - t_begin;
-    if (m == 0) return ASC_OK;
-    ASC_HIP(hipMemcpyAsync(t->od_list, t->h_list.p, 4 * m, hipMemcpyHostToDevice, ctx->s_copy));
-    ASC_HIP(hipMemcpyAsync(t->od_prefix, t->h_prefix.p, 8 * (m + 1), hipMemcpyHostToDevice, ctx->s_copy));
-    ASC_HIP(hipEventRecord(t->ev_meta, ctx->s_copy));
-    ASC_HIP(hipStreamWaitEvent(ctx->s_main, t->ev_meta, 0));
+    rc.node_sum += m;
+    rc.stream_rounds++;
+    ASC_HIP(hipMemcpyAsync(t->od_prefix, t->h_prefix.p, 8 * ((size_t)m + 1), hipMemcpyHostToDevice, s));
 
     PartArgs<EdgeT> p{};
     p.list = t->od_list;
     p.prefix = t->od_prefix;
-    p.n = (uint32_t)m;
+    p.n = m;
     p.tail_base = g->static_edges;
     const size_t EB = g->h.edge_bytes();
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
@@ -491,9 +611,9 @@ static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, 
         p.zc = (const EdgeT*)dptr;
         p.lo = 0;
         p.hi = T;
-        ASC_TRY(timed_launch(t, prof, KERN_PART, ctx->s_main, [&] {
-            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, ctx->s_main,
-                               a, level, p);
+        ASC_TRY(timed_launch(t, prof, KERN_PART, s, [&] {
+            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, s, a, level, p,
+                               list_sv);
         }));
         rc.parts += 1;
     } else {
@@ -509,15 +629,15 @@ static int stream_level(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, 
             if (q >= 2) ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
             ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[b].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
-            ASC_HIP(hipStreamWaitEvent(ctx->s_main, t->ev_copy[b], 0));
+            ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.part = (const EdgeT*)t->part[b];
             p.lo = lo;
             p.hi = hi;
-            ASC_TRY(timed_launch(t, prof, KERN_PART, ctx->s_main, [&] {
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0,
-                                   ctx->s_main, a, level, p);
+            ASC_TRY(timed_launch(t, prof, KERN_PART, s, [&] {
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0, s, a,
+                                   level, p, list_sv);
             }));
-            ASC_HIP(hipEventRecord(t->ev_kern[b], ctx->s_main));
+            ASC_HIP(hipEventRecord(t->ev_kern[b], s));
         }
         rc.parts += (uint32_t)np;
     }
@@ -537,21 +657,31 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     ASC_HIP(hipSetDevice(ctx->device));
     const int fill_grid = ctx->num_cus * 4;
     const int grid = ctx->num_cus * kBlocksPerCU;
+    const bool prof = ctx->cfg.profile != 0;
+    const uint32_t delta = ALGO == ALG_SSSP ? sssp_delta(g) : 0;
 
     // initial state
     ASC_HIP(hipMemsetAsync(t->slots, 0, sizeof(Slot) * 4, s));
+    ASC_HIP(hipMemsetAsync(t->piles, 0, sizeof(Piles), s));
     ASC_HIP(hipMemsetAsync(t->stats, 0, sizeof(DevStats), s));
-    if (ALGO == ALG_CC) {
-        hipLaunchKernelGGL(k_init_cc, dim3(fill_grid), dim3(256), 0, s, t->val, t->q[0], t->od[0], t->slots,
-                           (uint32_t)V, vs);
-    } else {
+    if (ALGO == ALG_BFS) {
         hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->val, V, (uint32_t)(V + 1));
-        if (ALGO == ALG_BFS)
-            hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->visited, (V + 31) / 32, 0u);
-        hipLaunchKernelGGL(k_init_source, dim3(1), dim3(1), 0, s, t->val, ALGO == ALG_BFS ? t->visited : nullptr,
-                           t->q[0], t->od[0], t->slots, src, vs);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->visited, (V + 31) / 32, 0u);
+        hipLaunchKernelGGL(k_init_source, dim3(1), dim3(1), 0, s, t->val, t->visited, (uint32_t*)nullptr, t->q[0],
+                           t->pend[0], t->slots, t->piles, src, vs);
+    } else {
+        for (uint32_t* p : {t->stamp, t->pstamp, t->fstamp})
+            hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, p, V, 0u);
+        hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->done, V, kNoValue);
+        if (ALGO == ALG_CC) {
+            hipLaunchKernelGGL(k_init_cc, dim3(fill_grid), dim3(256), 0, s, t->val, t->pstamp, t->q[0], t->pend[0],
+                               t->slots, t->piles, (uint32_t)V, vs);
+        } else {
+            hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->val, V, (uint32_t)(V + 1));
+            hipLaunchKernelGGL(k_init_source, dim3(1), dim3(1), 0, s, t->val, (uint32_t*)nullptr, t->pstamp, t->q[0],
+                               t->pend[0], t->slots, t->piles, src, vs);
+        }
     }
-    if (ALGO != ALG_BFS) hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->stamp, V, 0u);
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipStreamSynchronize(s));
 
@@ -560,6 +690,9 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     a.edges = (const EdgeT*)g->d_edges;
     a.val = t->val;
     a.stamp = t->stamp;
+    a.pstamp = t->pstamp;
+    a.fstamp = t->fstamp;
+    a.done = t->done;
     a.visited = t->visited;
     a.slots = t->slots;
     a.heavy_u = t->heavy_u;
@@ -568,50 +701,107 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     a.stats = t->stats;
     a.vs = vs;
     a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
+    a.theta = (ALGO == ALG_SSSP && delta) ? 1 + delta : 0xFFFFFFFFu;
+    a.pepoch = 1;
+    a.fepoch = 1;
+    uint32_t pcur = 0, fcur = 0;
 
-    uint32_t nr, no;
+    uint32_t nr, npend, nfar = 0;
     if (ALGO == ALG_CC) {
         nr = vs;
-        no = (uint32_t)(V - vs);
+        npend = (uint32_t)(V - vs);
     } else {
         nr = src < vs ? 1 : 0;
-        no = src < vs ? 0 : 1;
+        npend = src < vs ? 0 : 1;
     }
     RunCounters rc{};
-    const bool prof = ctx->cfg.profile != 0;
     t->timed.clear();
     t->ev_next = 0;
     const double t0 = now_ms();
     uint32_t level = 0;
-    for (;; level++) {
-        if (nr == 0 && no == 0) break;
-        rc.node_sum += (uint64_t)nr + no;
-        a.q_in = t->q[level & 1];
-        a.q_out = t->q[(level + 1) & 1];
-        a.od_out = t->od[(level + 1) & 1];
-        hipEvent_t e0 = nullptr;
-        if (!prof) {
-            e0 = next_event(t);
-            if (!e0) return ASC_ERR_DEVICE;
-            ASC_HIP(hipEventRecord(e0, s));
+    bool last_split_empty = false;
+    for (;;) {
+        enum { M_LEVEL, M_RES, M_STREAM, M_SPLIT } mode;
+        if (ALGO == ALG_BFS) {
+            if (nr == 0 && npend == 0) break;
+            mode = M_LEVEL;
+        } else if (nr > 0) {
+            mode = M_RES;
+        } else if (npend > 0) {
+            mode = M_STREAM;
+        } else if (nfar > 0) {
+            mode = M_SPLIT;
+        } else {
+            break;
         }
-        ASC_TRY(timed_launch(t, prof, KERN_LOCAL, s, [&] {
-            hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
-        }));
-        ASC_TRY(timed_launch(t, prof, KERN_HEAVY, s, [&] {
-            hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
-        }));
-        if (!prof) {
-            hipEvent_t e1 = next_event(t);
-            if (!e1) return ASC_ERR_DEVICE;
-            ASC_HIP(hipEventRecord(e1, s));
-            t->timed.push_back({-1, e0, e1});
+        const uint32_t* stream_list = nullptr;
+        uint32_t nstream = 0;
+        if (mode == M_LEVEL || mode == M_STREAM) {
+            if (npend > 0) {  // the pile filled so far is streamed; new entries go to the other one
+                stream_list = t->pend[pcur];
+                nstream = npend;
+                pcur ^= 1;
+                a.pepoch++;
+                hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_pend[pcur], 0u);
+            }
         }
-        if (no > 0) ASC_TRY((stream_level<ALGO, OffT, EdgeT>(g, a, level, no, grid, rc)));
-        ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + ((level + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
+        a.pend_out = t->pend[pcur];
+        a.pend_count = &t->piles->n_pend[pcur];
+        a.far_out = t->far[fcur];
+        a.far_count = &t->piles->n_far[fcur];
+        if (mode == M_SPLIT) {
+            // advance the band; entries still beyond it move to the other far pile
+            const uint64_t th = (uint64_t)(last_split_empty ? std::max(a.theta, t->h_piles[0].min_far) : a.theta) + delta;
+            a.theta = (uint32_t)std::min<uint64_t>(th, 0xFFFFFFFFull);
+            const uint32_t old = fcur;
+            fcur ^= 1;
+            a.fepoch++;
+            a.far_out = t->far[fcur];
+            a.far_count = &t->piles->n_far[fcur];
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_far[fcur], 0u);
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->min_far, 0xFFFFFFFFu);
+            hipLaunchKernelGGL((k_split_far<OffT, EdgeT>),
+                               dim3(std::min<uint32_t>((nfar + kBlock - 1) / kBlock, (uint32_t)grid)), dim3(kBlock), 0,
+                               s, a, t->far[old], nfar, level, t->q[level & 1], t->piles);
+            ASC_HIP(hipGetLastError());
+            rc.splits++;
+        } else {
+            // one step: expand the resident frontier (possibly empty), then
+            // stream the pile
+            rc.node_sum += nr;
+            a.q_in = t->q[level & 1];
+            a.q_out = t->q[(level + 1) & 1];
+            hipEvent_t e0 = nullptr;
+            if (!prof) {
+                e0 = next_event(t);
+                if (!e0) return ASC_ERR_DEVICE;
+                ASC_HIP(hipEventRecord(e0, s));
+            }
+            ASC_TRY(timed_launch(t, prof, KERN_LOCAL, s, [&] {
+                hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(nr ? grid : 1), dim3(kBlock), 0, s, a,
+                                   level);
+            }));
+            if (nr) {
+                ASC_TRY(timed_launch(t, prof, KERN_HEAVY, s, [&] {
+                    hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+                }));
+            }
+            if (!prof) {
+                hipEvent_t e1 = next_event(t);
+                if (!e1) return ASC_ERR_DEVICE;
+                ASC_HIP(hipEventRecord(e1, s));
+                t->timed.push_back({-1, e0, e1});
+            }
+            if (nstream) ASC_TRY((stream_pile<ALGO, OffT, EdgeT>(g, a, level, stream_list, nstream, grid, rc)));
+            level++;
+        }
+        ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + (level & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
         nr = t->h_slot[0].n_res;
-        no = t->h_slot[0].n_od;
+        npend = t->h_piles[0].n_pend[pcur];
+        nfar = ALGO == ALG_SSSP ? t->h_piles[0].n_far[fcur] : 0;
+        last_split_empty = mode == M_SPLIT && nr == 0 && npend == 0;
     }
     const double t1 = now_ms();
     double static_ms = 0;

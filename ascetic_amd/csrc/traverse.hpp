@@ -15,14 +15,24 @@ constexpr uint32_t kHeavyCap = 1u << 20;        // heavy list entries per level
 constexpr int kTile = 1024;                     // edges per tile in edge-balanced kernels
 constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
 constexpr int kBlocksPerCU = 8;
+constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
 
 // Per-level counters, in a ring of four: level L reads slot L%4, appends to
 // slot (L+1)%4 and clears slot (L+2)%4.
 struct Slot {
     uint32_t n_res;                 // resident frontier entries
-    uint32_t n_od;                  // on-demand frontier entries
+    uint32_t pad0;
     unsigned long long heavy;       // heavy list (count << 44 | edges)
     unsigned long long pad[2];
+};
+
+// Counters of the on-demand pile and the far pile (double-buffered).
+struct Piles {
+    uint32_t n_pend[2];
+    uint32_t n_far[2];
+    uint32_t n_list;                // entries kept by the pile filter
+    uint32_t min_far;               // smallest value left in the far pile
+    uint32_t pad[2];
 };
 
 struct DevStats {
@@ -41,11 +51,17 @@ struct TravArgs {
     const OffT* off;                // V+1 offsets
     const EdgeT* edges;             // static edge prefix
     uint32_t* val;
-    uint32_t* stamp;                // SSSP/CC: level a vertex was last queued for
+    uint32_t* stamp;                // SSSP/CC: level a resident vertex was last queued for
+    uint32_t* pstamp;               // SSSP/CC: on-demand pile epoch a vertex was last added to
+    uint32_t* fstamp;               // SSSP: far pile epoch a vertex was last added to
+    uint32_t* done;                 // SSSP/CC: value a vertex was last expanded with
     uint32_t* visited;              // BFS bitmap
     const uint32_t* q_in;
     uint32_t* q_out;
-    uint32_t* od_out;
+    uint32_t* pend_out;             // on-demand pile being filled
+    uint32_t* pend_count;
+    uint32_t* far_out;              // far pile being filled
+    uint32_t* far_count;
     Slot* slots;
     uint32_t* heavy_u;
     uint32_t* heavy_sv;
@@ -53,6 +69,8 @@ struct TravArgs {
     DevStats* stats;
     uint32_t vs;                    // vertices [0, vs) are resident
     uint32_t heavy_thr;
+    uint32_t theta;                 // SSSP near/far boundary (UINT_MAX: everything near)
+    uint32_t pepoch, fepoch;
 };
 
 template <class EdgeT>
@@ -69,26 +87,32 @@ struct PartArgs {
 struct RunCounters {
     double pre_ms = 0, overload_ms = 0, fill_ms = 0;
     uint64_t node_sum = 0, streamed_edges = 0;
-    uint32_t parts = 0;
+    uint32_t parts = 0, stream_rounds = 0, splits = 0;
 };
 
 struct TravState {
     uint64_t V = 0;
     uint32_t* val = nullptr;
     uint32_t* stamp = nullptr;
+    uint32_t* pstamp = nullptr;
+    uint32_t* fstamp = nullptr;
+    uint32_t* done = nullptr;
     uint32_t* visited = nullptr;
     uint32_t* q[2] = {nullptr, nullptr};
-    uint32_t* od[2] = {nullptr, nullptr};
+    uint32_t* pend[2] = {nullptr, nullptr};
+    uint32_t* far[2] = {nullptr, nullptr};
     uint32_t* heavy_u = nullptr;
     uint32_t* heavy_sv = nullptr;
     unsigned long long* heavy_base = nullptr;
     uint32_t* od_list = nullptr;
     unsigned long long* od_prefix = nullptr;
     Slot* slots = nullptr;
+    Piles* piles = nullptr;
     DevStats* stats = nullptr;
     void* part[2] = {nullptr, nullptr};
     uint64_t part_edges = 0;
     HostBuf<Slot> h_slot;
+    HostBuf<Piles> h_piles;
     HostBuf<DevStats> h_stats;
     HostBuf<uint32_t> h_od, h_list;
     HostBuf<unsigned long long> h_prefix;

@@ -63,6 +63,7 @@ def main():
     ap.add_argument("--budget-frac", type=float, default=0.4)
     ap.add_argument("--part-mb", type=int, default=256)
     ap.add_argument("--stream-mode", choices=["gather", "zerocopy"], default="gather")
+    ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     args = ap.parse_args()
@@ -87,7 +88,8 @@ def main():
     s_e, s_o = 8, 8
     budget = int(args.budget_frac * E * s_e)
     mode = _abi.STREAM_GATHER if args.stream_mode == "gather" else _abi.STREAM_ZEROCOPY
-    ctx = A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, stream_mode=mode, profile=True)
+    ctx = A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, stream_mode=mode, profile=True,
+                    sssp_delta=args.delta)
     g = ctx.from_arrays(off, ew)
     info = g.info()
     log(f"plan: static edges {info['static_edges']} ({info['static_edges'] / E:.1%}), static vertices "
@@ -137,7 +139,7 @@ def main():
             "V": V, "E_directed": E, "source": src, "offsets": "u64", "edge_record": "{u32 dst, u32 w}",
             "edge_budget_bytes": budget, "static_edges": info["static_edges"],
             "ondemand_edges": info["ondemand_edges"], "part_bytes": args.part_mb << 20,
-            "stream_mode": args.stream_mode,
+            "stream_mode": args.stream_mode, "sssp_delta": args.delta,
             "iterations": st0["iterations"], "reached": st0["reached"],
         },
         "roofline": {
