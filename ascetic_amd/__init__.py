@@ -18,7 +18,7 @@ from . import _abi
 from ._abi import AscError, check  # noqa: F401
 
 __all__ = [
-    "Context", "Graph", "HostGraph", "generate", "rmat", "erdos_renyi", "write_graph",
+    "Context", "Graph", "HostGraph", "generate", "read_graph", "rmat", "erdos_renyi", "write_graph",
     "plan_partition", "split_parts", "device_count", "bfs_opt", "cc_opt", "sssp_opt", "pr_opt",
     "AscError",
 ]
@@ -110,6 +110,13 @@ def generate(type=0, scale=10, num_vertices=0, num_samples=0, edge_factor=16, a=
     p.csc, p.threads = int(csc), threads
     h = _abi.HostCsr()
     check(lib.asc_generate(C.byref(p), C.byref(h)), "asc_generate")
+    return HostGraph(h)
+
+
+def read_graph(path, fmt=_abi.FMT_AUTO):
+    """Reads a graph file into host memory (no device needed)."""
+    h = _abi.HostCsr()
+    check(_abi.load().asc_read_graph(path.encode(), fmt, C.byref(h)), f"asc_read_graph({path})")
     return HostGraph(h)
 
 

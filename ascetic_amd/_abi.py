@@ -235,6 +235,7 @@ SIGNATURES = {
     "asc_split_parts": (C.c_int, [u64p, C.c_uint64, C.c_uint64, u64p, C.c_uint64, u64p]),
     "asc_gen_params_init": (None, [C.POINTER(GenParams)]),
     "asc_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(HostCsr)]),
+    "asc_read_graph": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(HostCsr)]),
     "asc_host_csr_free": (None, [C.POINTER(HostCsr)]),
     "asc_host_csr_view": (None, [C.POINTER(HostCsr), C.POINTER(CsrView)]),
     "asc_graph_adopt": (C.c_int, [P, C.POINTER(HostCsr), C.c_int, PP]),

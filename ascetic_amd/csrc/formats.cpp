@@ -282,3 +282,20 @@ int write_graph_file(const char* path, int format, const asc_csr_view* v) {
 extern "C" int asc_write_graph(const char* path, int format, const asc_csr_view* view) {
     return asc::write_graph_file(path, format, view);
 }
+
+extern "C" int asc_read_graph(const char* path, int format, asc_host_csr* out) {
+    if (!out) return ASC_ERR_INVALID;
+    std::memset(out, 0, sizeof(*out));
+    asc::HostCSR h;
+    ASC_TRY(asc::read_graph_file(path, format, h));
+    out->num_vertices = h.V;
+    out->num_edges = h.E;
+    out->kind = h.kind;
+    out->off = h.off.p;
+    out->edges = h.edges.p;
+    out->out_degree = h.outdeg.p;
+    h.off.p = nullptr;  // ownership moves to the caller (asc_host_csr_free)
+    h.edges.p = nullptr;
+    h.outdeg.p = nullptr;
+    return ASC_OK;
+}

@@ -291,6 +291,9 @@ typedef struct {
 
 /* Generates a graph: self-loops and duplicate edges removed, adjacency sorted. */
 int asc_generate(const asc_gen_params* p, asc_host_csr* out);
+/* Reads a graph file into host memory only (no device needed); same
+ * validation as asc_graph_load (GraphMeta.cuh:151-171). */
+int asc_read_graph(const char* path, int format, asc_host_csr* out);
 void asc_host_csr_free(asc_host_csr* h);
 /* View over a host CSR (for asc_graph_from_csr / asc_write_graph). */
 void asc_host_csr_view(const asc_host_csr* h, asc_csr_view* v);
