@@ -307,6 +307,64 @@ class Graph:
         return r, None, st.as_dict()
 
 
+class Server:
+    """Continuous batching of BFS / SSSP queries over a resident graph.
+
+    submit() is thread-safe and returns a ticket; when `out` is given (a
+    uint32 array of V entries) it is filled before the query completes.
+    wait(ticket) blocks until completion and returns the query's stats."""
+
+    BFS = _abi.QUERY_BFS
+    SSSP = _abi.QUERY_SSSP
+
+    def __init__(self, graph, lanes=64, state_budget_bytes=0, queue_capacity=4096):
+        lib = _abi.load()
+        o = _abi.ServerOpts()
+        lib.asc_server_opts_init(C.byref(o))
+        o.lanes, o.state_budget_bytes, o.queue_capacity = lanes, state_budget_bytes, queue_capacity
+        h = C.c_void_p()
+        check(lib.asc_server_create(graph._h, C.byref(o), C.byref(h)), "asc_server_create")
+        self._h = h
+        self.graph = graph
+        self._outs = {}
+
+    def submit(self, kind, source, out=None):
+        t = C.c_uint64()
+        ptr = None
+        if out is not None:
+            assert out.dtype == np.uint32 and out.flags["C_CONTIGUOUS"]
+            ptr = _u32p(out)
+        check(_abi.load().asc_submit(self._h, kind, source, ptr, C.byref(t)), "asc_submit")
+        if out is not None:
+            self._outs[t.value] = out  # keep the buffer alive until completion
+        return t.value
+
+    def wait(self, ticket):
+        qs = _abi.QStats()
+        check(_abi.load().asc_wait(self._h, ticket, C.byref(qs)), "asc_wait")
+        self._outs.pop(ticket, None)
+        return {n: getattr(qs, n) for n, _ in qs._fields_}
+
+    def drain(self):
+        check(_abi.load().asc_server_drain(self._h), "asc_server_drain")
+
+    def stats(self):
+        s = _abi.ServerStats()
+        check(_abi.load().asc_server_get_stats(self._h, C.byref(s)), "asc_server_get_stats")
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def close(self):
+        if self._h:
+            _abi.load().asc_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 # ---- reference-named drivers (CalculateOpt.cuh:11-14) ----------------------
 
 def _driver(path, advise_rate, fmt, verbose, run):

@@ -183,7 +183,18 @@ class ServerOpts(C.Structure):
         ("lanes", C.c_int),
         ("state_budget_bytes", C.c_uint64),
         ("queue_capacity", C.c_int),
-        ("keep_results_on_device", C.c_int),
+        ("pad", C.c_int),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64),
+        ("completed", C.c_uint64),
+        ("steps", C.c_uint64),
+        ("lanes_bfs", C.c_int),
+        ("lanes_sssp", C.c_int),
+        ("busy_ms", C.c_double),
     ]
 
 
@@ -242,9 +253,10 @@ SIGNATURES = {
     "asc_server_opts_init": (None, [C.POINTER(ServerOpts)]),
     "asc_server_create": (C.c_int, [P, C.POINTER(ServerOpts), PP]),
     "asc_server_destroy": (C.c_int, [P]),
-    "asc_submit": (C.c_int, [P, C.c_int, C.c_uint32, u64p]),
-    "asc_wait": (C.c_int, [P, C.c_uint64, u32p, C.POINTER(QStats)]),
+    "asc_submit": (C.c_int, [P, C.c_int, C.c_uint32, u32p, u64p]),
+    "asc_wait": (C.c_int, [P, C.c_uint64, C.POINTER(QStats)]),
     "asc_server_drain": (C.c_int, [P]),
+    "asc_server_get_stats": (C.c_int, [P, C.POINTER(ServerStats)]),
     "asc_autotune": (C.c_int, [P, C.c_double, C.POINTER(Tuning)]),
     "asc_tuning_save": (C.c_int, [C.c_char_p, P, C.POINTER(Tuning)]),
     "asc_tuning_load": (C.c_int, [C.c_char_p, P, C.POINTER(Tuning)]),

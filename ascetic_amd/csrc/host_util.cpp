@@ -162,7 +162,7 @@ void asc_server_opts_init(asc_server_opts* o) {
     o->lanes = 64;
     o->state_budget_bytes = 0;
     o->queue_capacity = 4096;
-    o->keep_results_on_device = 0;
+    o->pad = 0;
 }
 
 }  // extern "C"

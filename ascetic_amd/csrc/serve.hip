@@ -1,13 +1,793 @@
-// Batched query serving (placeholder until the multi-source engine lands).
+// Batched query serving: continuous batching of BFS and SSSP queries over a
+// resident graph.  No counterpart in the reference, which runs one traversal
+// per process (main.cu:4-22).
+//
+// Two engines, each with up to 64 lanes (one query per lane):
+//   BFS   bitset-per-vertex multi-source BFS: seen[v] and the frontier words
+//         X[v] / Y[v] are 64-bit, bit q = lane q, so one scan of an edge
+//         (u, v) advances every lane whose frontier holds u:
+//         new = X[u] & ~seen[v].  Levels are kept lane-major (lev[q][v]), so
+//         a finished query's result is one contiguous copy.
+//   SSSP  distance lanes: dist[v][q] vertex-major, one wavefront per edge
+//         with wave lane q relaxing query q, so a relaxation reads and
+//         updates 4*lanes contiguous bytes; improved lanes are ORed into the
+//         destination's 64-bit frontier word.
+// One engine step advances every admitted query by one frontier round, using
+// the same edge-balanced workgroup expansion and grid-cooperative heavy list
+// as the single-query engine.  A lane whose query produced no new frontier bit
+// in a step is finished; its result is copied out (if requested) and the lane
+// is free for the next waiting query at the next step boundary.  Lanes are
+// granted up to the device-memory budget (4 V bytes of per-query state each).
+// Every query's values equal those of the single-query engine: BFS levels are
+// level-synchronous per lane, and SSSP lanes converge to the unique distances.
+#include <algorithm>
+#include <condition_variable>
+#include <deque>
+#include <type_traits>
+#include <unordered_map>
+
+#include "device_common.hpp"
 #include "internal.hpp"
+#include "traverse.hpp"
+
+namespace asc {
+
+constexpr int kMaxLanes = 64;
+
+struct SeedArgs {
+    uint32_t n;
+    uint32_t lane[kMaxLanes];
+    uint32_t src[kMaxLanes];
+};
+
+template <class OffT, class EdgeT>
+struct BatchArgs {
+    const OffT* off;
+    const EdgeT* edges;
+    unsigned long long* X;           // frontier bits of this step (read and cleared)
+    unsigned long long* Y;           // frontier bits of the next step
+    unsigned long long* seen;        // BFS
+    uint32_t* lev;                   // BFS: lev[q * V + v]
+    uint32_t* dist;                  // SSSP: dist[v * stride + q]
+    const uint32_t* q_in;
+    uint32_t* q_out;
+    Slot* slots;                     // pad[0] of slot (s+1)%4: lanes that advanced in step s
+    uint32_t* heavy_u;
+    unsigned long long* heavy_bits;
+    unsigned long long* heavy_base;
+    const uint32_t* s0;              // step at which each lane was seeded
+    uint64_t V;
+    uint32_t stride;                 // SSSP row length (lanes)
+    uint32_t heavy_thr;
+};
+
+// -- BFS lanes ------------------------------------------------------------------
+
+template <class OffT, class EdgeT>
+__device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uint32_t v, unsigned long long bits,
+                                            uint32_t step, unsigned long long& act) {
+    unsigned long long nw = bits & ~a.seen[v];
+    if (!nw) return false;
+    const unsigned long long old = atomicOr(a.seen + v, nw);
+    nw &= ~old;
+    if (!nw) return false;
+    act |= nw;
+    for (unsigned long long m = nw; m; m &= m - 1) {
+        const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+        a.lev[(size_t)q * a.V + v] = step - a.s0[q] + 2;
+    }
+    return atomicOr(a.Y + v, nw) == 0;
+}
+
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a, uint32_t step) {
+    __shared__ uint32_t s_pref[kBlock];
+    __shared__ OffT s_beg[kBlock];
+    __shared__ unsigned long long s_bits[kBlock];
+    __shared__ uint32_t s_tmp[kWavesPerBlock];
+    __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    Slot* cur = a.slots + (step & 3);
+    Slot* nxt = a.slots + ((step + 1) & 3);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Slot* z = a.slots + ((step + 2) & 3);
+        z->n_res = 0;
+        z->heavy = 0;
+        z->pad[0] = 0;
+    }
+    const uint32_t n = cur->n_res;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    WaveStage<kStageCap> st{s_stage[w], 0};
+    unsigned long long act = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t deg = 0;
+        OffT beg = 0;
+        unsigned long long bits = 0;
+        if (i < n) {
+            const uint32_t u = a.q_in[i];
+            bits = a.X[u];
+            a.X[u] = 0;
+            beg = a.off[u];
+            const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+            if (d >= a.heavy_thr) {
+                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
+                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
+                a.heavy_u[idx] = u;
+                a.heavy_bits[idx] = bits;
+                a.heavy_base[idx] = p & kHeavyMask;
+            } else {
+                deg = (uint32_t)d;
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = block_scan_excl(deg, s_tmp, total);
+        s_pref[tid] = pre;
+        s_beg[tid] = beg;
+        s_bits[tid] = bits;
+        __syncthreads();
+        const uint32_t cnt = min(n - base, (uint32_t)kBlock);
+        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
+            const uint32_t e = eb + lane;
+            bool push = false;
+            uint32_t v = 0;
+            if (e < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t stp = kBlock / 2; stp > 0; stp >>= 1)
+                    if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
+                v = edge_dst(a.edges[s_beg[k] + (OffT)(e - s_pref[k])]);
+                push = msbfs_visit(a, v, s_bits[k], step, act);
+            }
+            st.push(push, v, a.q_out, &nxt->n_res);
+        }
+        __syncthreads();
+    }
+    st.flush(a.q_out, &nxt->n_res);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
+    if (lane == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a, uint32_t step) {
+    constexpr int kWin = 8;
+    __shared__ unsigned long long s_hb[kWin], s_hbits[kWin];
+    __shared__ OffT s_hoff[kWin];
+    __shared__ uint32_t s_k0;
+    __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    Slot* cur = a.slots + (step & 3);
+    Slot* nxt = a.slots + ((step + 1) & 3);
+    const unsigned long long h = cur->heavy;
+    const uint32_t nh = (uint32_t)(h >> kHeavyShift);
+    const uint64_t total = h & kHeavyMask;
+    if (nh == 0) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    WaveStage<kStageCap> st{s_stage[w], 0};
+    unsigned long long act = 0;
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
+        if (tid == 0) {
+            uint32_t lo = 0, hi = nh - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+            }
+            s_k0 = lo;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k0;
+        const uint32_t nwin = min(nh - k0, (uint32_t)kWin);
+        if (tid < nwin) {
+            s_hb[tid] = a.heavy_base[k0 + tid];
+            s_hoff[tid] = a.off[a.heavy_u[k0 + tid]];
+            s_hbits[tid] = a.heavy_bits[k0 + tid];
+        }
+        __syncthreads();
+        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
+        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
+            const uint32_t rr = r + lane;
+            bool push = false;
+            uint32_t v = 0;
+            if (rr < tlen) {
+                const uint64_t e = t0 + rr;
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
+                    if (j < nwin && s_hb[j] <= e) k = j;
+                v = edge_dst(a.edges[s_hoff[k] + (OffT)(e - s_hb[k])]);
+                push = msbfs_visit(a, v, s_hbits[k], step, act);
+            }
+            st.push(push, v, a.q_out, &nxt->n_res);
+        }
+        __syncthreads();
+    }
+    st.flush(a.q_out, &nxt->n_res);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
+    if (lane == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+// -- SSSP lanes -----------------------------------------------------------------
+
+// One wavefront relaxes edge (u, v, w) for every lane in `bits`.
+template <class OffT>
+__device__ __forceinline__ void mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
+                                           unsigned long long bits, unsigned long long& act, bool& push) {
+    const uint32_t lane = lane_id();
+    bool imp = false;
+    if (lane < a.stride && ((bits >> lane) & 1)) {
+        const uint32_t cand = a.dist[(size_t)u * a.stride + lane] + wgt;
+        uint32_t* dv = a.dist + (size_t)v * a.stride + lane;
+        if (cand < *dv) imp = atomicMin(dv, cand) > cand;
+    }
+    const unsigned long long m = wave_ballot(imp);
+    act |= m;
+    push = false;
+    if (m && lane == 0) push = atomicOr(a.Y + v, m) == 0;
+}
+
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a, uint32_t step) {
+    __shared__ uint32_t s_pref[kBlock];
+    __shared__ OffT s_beg[kBlock];
+    __shared__ uint32_t s_u[kBlock];
+    __shared__ unsigned long long s_bits[kBlock];
+    __shared__ uint32_t s_tmp[kWavesPerBlock];
+    __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    Slot* cur = a.slots + (step & 3);
+    Slot* nxt = a.slots + ((step + 1) & 3);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Slot* z = a.slots + ((step + 2) & 3);
+        z->n_res = 0;
+        z->heavy = 0;
+        z->pad[0] = 0;
+    }
+    const uint32_t n = cur->n_res;
+    const uint32_t tid = threadIdx.x, w = wave_id();
+    WaveStage<kStageCap> st{s_stage[w], 0};
+    unsigned long long act = 0;
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t deg = 0, u = 0;
+        OffT beg = 0;
+        unsigned long long bits = 0;
+        if (i < n) {
+            u = a.q_in[i];
+            bits = a.X[u];
+            a.X[u] = 0;
+            beg = a.off[u];
+            const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+            if (d >= a.heavy_thr) {
+                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
+                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
+                a.heavy_u[idx] = u;
+                a.heavy_bits[idx] = bits;
+                a.heavy_base[idx] = p & kHeavyMask;
+            } else {
+                deg = (uint32_t)d;
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = block_scan_excl(deg, s_tmp, total);
+        s_pref[tid] = pre;
+        s_beg[tid] = beg;
+        s_u[tid] = u;
+        s_bits[tid] = bits;
+        __syncthreads();
+        const uint32_t cnt = min(n - base, (uint32_t)kBlock);
+        // wave-uniform edge index: each wavefront relaxes one edge for all lanes
+        for (uint32_t e = w; e < total; e += kWavesPerBlock) {
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t stp = kBlock / 2; stp > 0; stp >>= 1)
+                if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
+            const WEdge ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
+            bool push;
+            mssp_relax(a, s_u[k], ed.dst, ed.w, s_bits[k], act, push);
+            st.push(push, ed.dst, a.q_out, &nxt->n_res);
+        }
+        __syncthreads();
+    }
+    st.flush(a.q_out, &nxt->n_res);
+    if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a, uint32_t step) {
+    constexpr int kWin = 8;
+    __shared__ unsigned long long s_hb[kWin], s_hbits[kWin];
+    __shared__ OffT s_hoff[kWin];
+    __shared__ uint32_t s_hu[kWin];
+    __shared__ uint32_t s_k0;
+    __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    Slot* cur = a.slots + (step & 3);
+    Slot* nxt = a.slots + ((step + 1) & 3);
+    const unsigned long long h = cur->heavy;
+    const uint32_t nh = (uint32_t)(h >> kHeavyShift);
+    const uint64_t total = h & kHeavyMask;
+    if (nh == 0) return;
+    const uint32_t tid = threadIdx.x, w = wave_id();
+    WaveStage<kStageCap> st{s_stage[w], 0};
+    unsigned long long act = 0;
+    constexpr uint32_t kSTile = 256;  // edges per workgroup tile (one per wavefront step)
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kSTile; t0 < total; t0 += (uint64_t)gridDim.x * kSTile) {
+        if (tid == 0) {
+            uint32_t lo = 0, hi = nh - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+            }
+            s_k0 = lo;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k0;
+        const uint32_t nwin = min(nh - k0, (uint32_t)kWin);
+        if (tid < nwin) {
+            s_hb[tid] = a.heavy_base[k0 + tid];
+            s_hu[tid] = a.heavy_u[k0 + tid];
+            s_hoff[tid] = a.off[a.heavy_u[k0 + tid]];
+            s_hbits[tid] = a.heavy_bits[k0 + tid];
+        }
+        __syncthreads();
+        const uint32_t tlen = (uint32_t)min((uint64_t)kSTile, total - t0);
+        for (uint32_t r = w; r < tlen; r += kWavesPerBlock) {
+            const uint64_t e = t0 + r;
+            uint32_t k = 0;
+#pragma unroll
+            for (uint32_t j = 1; j < (uint32_t)kWin; j++)
+                if (j < nwin && s_hb[j] <= e) k = j;
+            const WEdge ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
+            bool push;
+            mssp_relax(a, s_hu[k], ed.dst, ed.w, s_hbits[k], act, push);
+            st.push(push, ed.dst, a.q_out, &nxt->n_res);
+        }
+        __syncthreads();
+    }
+    st.flush(a.q_out, &nxt->n_res);
+    if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+// -- admission ------------------------------------------------------------------
+
+__global__ void k_clear_seen_bits(unsigned long long* seen, uint64_t V, unsigned long long mask) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        seen[v] &= ~mask;
+}
+
+__global__ void k_reset_dist(uint32_t* dist, uint64_t V, uint32_t stride, unsigned long long mask, uint32_t inf) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        for (unsigned long long m = mask; m; m &= m - 1) dist[v * stride + (__ffsll((long long)m) - 1)] = inf;
+}
+
+template <class OffT, class EdgeT>
+__global__ void k_seed(BatchArgs<OffT, EdgeT> a, SeedArgs sa, uint32_t step, uint32_t* s0, int bfs) {
+    const uint32_t i = threadIdx.x;
+    if (i >= sa.n) return;
+    const uint32_t q = sa.lane[i], src = sa.src[i];
+    const unsigned long long bit = 1ull << q;
+    s0[q] = step;
+    if (bfs) {
+        atomicOr(a.seen + src, bit);
+        a.lev[(size_t)q * a.V + src] = 1;
+    } else {
+        a.dist[(size_t)src * a.stride + q] = 1;
+    }
+    if (atomicOr(a.X + src, bit) == 0) {
+        const uint32_t pos = atomicAdd(&a.slots[step & 3].n_res, 1u);
+        ((uint32_t*)a.q_in)[pos] = src;
+    }
+}
+
+__global__ void k_lane_fill(uint32_t* p, uint64_t n, uint32_t v) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_extract_lane(const uint32_t* dist, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = dist[v * stride + lane];
+}
+
+// -- host -----------------------------------------------------------------------
+
+struct SQuery {
+    uint64_t ticket = 0;
+    int kind = 0;
+    uint32_t src = 0;
+    uint32_t* out = nullptr;
+    double t_sub = 0, t_adm = 0, t_done = 0;
+    uint32_t lane = 0, steps = 0;
+    bool done = false;
+};
+
+struct Engine {
+    int kind = 0;
+    int lanes = 0;
+    uint64_t V = 0;
+    hipStream_t stream = nullptr;
+    unsigned long long* bits[2] = {nullptr, nullptr};
+    unsigned long long* seen = nullptr;
+    uint32_t* lanes_state = nullptr;  // lev (BFS) or dist (SSSP)
+    uint32_t* q[2] = {nullptr, nullptr};
+    Slot* slots = nullptr;
+    uint32_t* heavy_u = nullptr;
+    unsigned long long* heavy_bits = nullptr;
+    unsigned long long* heavy_base = nullptr;
+    uint32_t* s0 = nullptr;
+    uint32_t* staging = nullptr;      // SSSP result column
+    HostBuf<Slot> h_slot;
+    SQuery* lane_q[kMaxLanes] = {};
+    unsigned long long busy = 0;
+    uint32_t step = 0;
+    bool launched = false;
+};
+
+}  // namespace asc
+
+struct asc_server {
+    asc_graph* g = nullptr;
+    asc_server_opts opts{};
+    asc::Engine eng[2];
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<asc::SQuery*> pending[2];
+    std::unordered_map<uint64_t, asc::SQuery*> tickets;
+    uint64_t next_ticket = 1, submitted = 0, completed = 0, steps = 0, outstanding = 0;
+    double busy_ms = 0;
+    bool stop = false;
+    int error = ASC_OK;
+};
+
+namespace asc {
+
+static void engine_free(Engine& e) {
+    auto f = [](void* p) {
+        if (p) (void)hipFree(p);
+    };
+    f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
+    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging);
+    if (e.stream) (void)hipStreamDestroy(e.stream);
+    e.stream = nullptr;
+}
+
+static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
+    e.kind = kind;
+    e.lanes = lanes;
+    e.V = V;
+    if (lanes <= 0) return ASC_OK;
+    ASC_HIP(hipStreamCreateWithFlags(&e.stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; i++) {
+        ASC_HIP(hipMalloc(&e.bits[i], 8 * V));
+        ASC_HIP(hipMemsetAsync(e.bits[i], 0, 8 * V, e.stream));
+        ASC_HIP(hipMalloc(&e.q[i], 4 * V));
+    }
+    if (kind == ASC_QUERY_BFS) {
+        ASC_HIP(hipMalloc(&e.seen, 8 * V));
+        ASC_HIP(hipMemsetAsync(e.seen, 0, 8 * V, e.stream));
+    } else {
+        ASC_HIP(hipMalloc(&e.staging, 4 * V));
+    }
+    hipError_t r = hipMalloc(&e.lanes_state, 4 * V * (size_t)lanes);
+    if (r != hipSuccess) {
+        set_last_error("serving state of %d lanes x %llu vertices does not fit", lanes, (unsigned long long)V);
+        return ASC_ERR_NOMEM;
+    }
+    ASC_HIP(hipMalloc(&e.slots, sizeof(Slot) * 4));
+    ASC_HIP(hipMemsetAsync(e.slots, 0, sizeof(Slot) * 4, e.stream));
+    ASC_HIP(hipMalloc(&e.heavy_u, 4 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&e.heavy_bits, 8 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&e.heavy_base, 8 * (size_t)kHeavyCap));
+    ASC_HIP(hipMalloc(&e.s0, 4 * kMaxLanes));
+    ASC_TRY(e.h_slot.alloc(1, true));
+    ASC_HIP(hipStreamSynchronize(e.stream));
+    return ASC_OK;
+}
+
+template <class OffT, class EdgeT>
+static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
+    BatchArgs<OffT, EdgeT> a{};
+    a.off = (const OffT*)g->d_off;
+    a.edges = (const EdgeT*)g->d_edges;
+    a.X = e.bits[e.step & 1];
+    a.Y = e.bits[(e.step + 1) & 1];
+    a.seen = e.seen;
+    a.lev = e.kind == ASC_QUERY_BFS ? e.lanes_state : nullptr;
+    a.dist = e.kind == ASC_QUERY_SSSP ? e.lanes_state : nullptr;
+    a.q_in = e.q[e.step & 1];
+    a.q_out = e.q[(e.step + 1) & 1];
+    a.slots = e.slots;
+    a.heavy_u = e.heavy_u;
+    a.heavy_bits = e.heavy_bits;
+    a.heavy_base = e.heavy_base;
+    a.s0 = e.s0;
+    a.V = e.V;
+    a.stride = (uint32_t)e.lanes;
+    a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
+    return a;
+}
+
+// Resets and seeds newly admitted lanes, then launches one step.
+template <class OffT, class EdgeT>
+static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
+    const int grid = g->ctx->num_cus * kBlocksPerCU;
+    const uint32_t unreached = (uint32_t)(e.V + 1);
+    auto a = engine_args<OffT, EdgeT>(g, e);
+    if (!admitted.empty()) {
+        SeedArgs sa{};
+        unsigned long long mask = 0;
+        for (SQuery* q : admitted) {
+            sa.lane[sa.n] = q->lane;
+            sa.src[sa.n] = q->src;
+            sa.n++;
+            mask |= 1ull << q->lane;
+        }
+        if (e.kind == ASC_QUERY_BFS) {
+            hipLaunchKernelGGL(k_clear_seen_bits, dim3(grid), dim3(256), 0, e.stream, e.seen, e.V, mask);
+            for (SQuery* q : admitted)
+                hipLaunchKernelGGL(k_lane_fill, dim3(grid), dim3(256), 0, e.stream,
+                                   e.lanes_state + (size_t)q->lane * e.V, e.V, unreached);
+        } else {
+            hipLaunchKernelGGL(k_reset_dist, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
+                               (uint32_t)e.lanes, mask, unreached);
+        }
+        hipLaunchKernelGGL((k_seed<OffT, EdgeT>), dim3(1), dim3(kMaxLanes), 0, e.stream, a, sa, e.step, e.s0,
+                           e.kind == ASC_QUERY_BFS ? 1 : 0);
+        ASC_HIP(hipGetLastError());
+    }
+    if constexpr (std::is_same<EdgeT, WEdge>::value) {
+        if (e.kind == ASC_QUERY_SSSP) {
+            hipLaunchKernelGGL((k_mssp_local<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+            hipLaunchKernelGGL((k_mssp_heavy<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+        }
+    }
+    if (e.kind == ASC_QUERY_BFS) {
+        hipLaunchKernelGGL((k_msbfs_local<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+        hipLaunchKernelGGL((k_msbfs_heavy<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+    }
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipMemcpyAsync(e.h_slot.p, e.slots + ((e.step + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, e.stream));
+    e.launched = true;
+    return ASC_OK;
+}
+
+// After a step: finished lanes copy their results out and are released.
+static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
+    if (!e.launched) return ASC_OK;
+    ASC_HIP(hipStreamSynchronize(e.stream));
+    e.launched = false;
+    const unsigned long long act = e.h_slot[0].pad[0];
+    for (int q = 0; q < e.lanes; q++) {
+        SQuery* qq = e.lane_q[q];
+        if (!qq) continue;
+        qq->steps++;
+        if (act & (1ull << q)) continue;
+        if (qq->out) {
+            if (e.kind == ASC_QUERY_BFS) {
+                ASC_HIP(hipMemcpyAsync(qq->out, e.lanes_state + (size_t)q * e.V, 4 * e.V, hipMemcpyDeviceToHost,
+                                       e.stream));
+            } else {
+                const int grid = s->g->ctx->num_cus * 4;
+                hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
+                                   (uint32_t)e.lanes, (uint32_t)q, e.staging);
+                ASC_HIP(hipMemcpyAsync(qq->out, e.staging, 4 * e.V, hipMemcpyDeviceToHost, e.stream));
+            }
+            ASC_HIP(hipStreamSynchronize(e.stream));
+        }
+        qq->t_done = now_ms();
+        e.lane_q[q] = nullptr;
+        e.busy &= ~(1ull << q);
+        done.push_back(qq);
+    }
+    e.step++;
+    return ASC_OK;
+}
+
+template <class OffT, class EdgeT>
+static void server_loop(asc_server* s) {
+    (void)hipSetDevice(s->g->ctx->device);
+    std::vector<SQuery*> admitted[2], done;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv_work.wait(lk, [&] {
+                return s->stop || !s->pending[0].empty() || !s->pending[1].empty() || s->eng[0].busy ||
+                       s->eng[1].busy;
+            });
+            if (s->stop && !s->eng[0].busy && !s->eng[1].busy && s->pending[0].empty() && s->pending[1].empty())
+                break;
+            const double now = now_ms();
+            for (int k = 0; k < 2; k++) {
+                Engine& e = s->eng[k];
+                admitted[k].clear();
+                for (int q = 0; q < e.lanes && !s->pending[k].empty(); q++) {
+                    if (e.busy & (1ull << q)) continue;
+                    SQuery* qq = s->pending[k].front();
+                    s->pending[k].pop_front();
+                    qq->lane = (uint32_t)q;
+                    qq->t_adm = now;
+                    e.lane_q[q] = qq;
+                    e.busy |= 1ull << q;
+                    admitted[k].push_back(qq);
+                }
+            }
+        }
+        const double t_step = now_ms();
+        int r = ASC_OK;
+        for (int k = 0; k < 2 && r == ASC_OK; k++)
+            if (s->eng[k].busy) {
+                if (k == ASC_QUERY_SSSP) {
+                    if constexpr (std::is_same<EdgeT, WEdge>::value)
+                        r = engine_launch<OffT, EdgeT>(s->g, s->eng[k], admitted[k]);
+                } else {
+                    r = engine_launch<OffT, EdgeT>(s->g, s->eng[k], admitted[k]);
+                }
+            }
+        done.clear();
+        for (int k = 0; k < 2 && r == ASC_OK; k++) r = engine_finish(s, s->eng[k], done);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->busy_ms += now_ms() - t_step;
+            s->steps++;
+            if (r != ASC_OK) {
+                // a device error fails every query in flight or waiting
+                s->error = r;
+                for (int k = 0; k < 2; k++) {
+                    for (int q = 0; q < s->eng[k].lanes; q++)
+                        if (s->eng[k].lane_q[q]) done.push_back(s->eng[k].lane_q[q]), s->eng[k].lane_q[q] = nullptr;
+                    s->eng[k].busy = 0;
+                    for (SQuery* qq : s->pending[k]) done.push_back(qq);
+                    s->pending[k].clear();
+                }
+            }
+            for (SQuery* qq : done) {
+                qq->done = true;
+                s->completed++;
+                s->outstanding--;
+            }
+        }
+        if (!done.empty()) s->cv_done.notify_all();
+    }
+}
+
+}  // namespace asc
+
+using namespace asc;
 
 extern "C" {
-int asc_server_create(asc_graph*, const asc_server_opts*, asc_server** out) {
-    if (out) *out = nullptr;
-    return ASC_ERR_UNSUPPORTED;
+
+int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** out) {
+    if (!g || !out) return ASC_ERR_INVALID;
+    *out = nullptr;
+    if (g->h.kind == ASC_KIND_CSC) return ASC_ERR_UNSUPPORTED;
+    if (g->static_vertices < g->h.V) {
+        set_last_error("serving needs a fully resident graph");
+        return ASC_ERR_UNSUPPORTED;
+    }
+    asc_server_opts o;
+    if (opts) o = *opts; else asc_server_opts_init(&o);
+    if (o.lanes <= 0 || o.lanes > kMaxLanes) o.lanes = kMaxLanes;
+    if (o.queue_capacity <= 0) o.queue_capacity = 4096;
+    ASC_HIP(hipSetDevice(g->ctx->device));
+    const uint64_t V = g->h.V;
+    size_t free_b = 0, total_b = 0;
+    ASC_HIP(hipMemGetInfo(&free_b, &total_b));
+    // fixed state per engine: two frontier words, seen word, queues, heavy list
+    const uint64_t fixed = 8 * V * 3 + 4 * V * 2 + 16 * (uint64_t)kHeavyCap + 4 * V;
+    uint64_t budget = o.state_budget_bytes ? o.state_budget_bytes : free_b / 2;
+    budget = std::min<uint64_t>(budget, free_b > 2 * fixed ? free_b - 2 * fixed : 0);
+    const bool weighted = g->h.kind == ASC_KIND_WCSR;
+    const uint64_t per_lane = 4 * V;
+    const int kinds = weighted ? 2 : 1;
+    int lanes = (int)std::min<uint64_t>((uint64_t)o.lanes, budget / kinds / per_lane);
+    if (lanes < 1) {
+        set_last_error("state budget %llu bytes admits no query (%llu bytes per lane)", (unsigned long long)budget,
+                       (unsigned long long)per_lane);
+        return ASC_ERR_NOMEM;
+    }
+    asc_server* s = new asc_server();
+    s->g = g;
+    s->opts = o;
+    int r = engine_init(s->eng[ASC_QUERY_BFS], ASC_QUERY_BFS, lanes, V);
+    if (r == ASC_OK) r = engine_init(s->eng[ASC_QUERY_SSSP], ASC_QUERY_SSSP, weighted ? lanes : 0, V);
+    if (r != ASC_OK) {
+        engine_free(s->eng[0]);
+        engine_free(s->eng[1]);
+        delete s;
+        return r;
+    }
+    const bool w = weighted, o4 = g->off_bytes == 4;
+    if (w) s->th = std::thread(o4 ? server_loop<uint32_t, WEdge> : server_loop<uint64_t, WEdge>, s);
+    else s->th = std::thread(o4 ? server_loop<uint32_t, uint32_t> : server_loop<uint64_t, uint32_t>, s);
+    *out = s;
+    return ASC_OK;
 }
-int asc_server_destroy(asc_server*) { return ASC_ERR_UNSUPPORTED; }
-int asc_submit(asc_server*, int, uint32_t, uint64_t*) { return ASC_ERR_UNSUPPORTED; }
-int asc_wait(asc_server*, uint64_t, uint32_t*, asc_qstats*) { return ASC_ERR_UNSUPPORTED; }
-int asc_server_drain(asc_server*) { return ASC_ERR_UNSUPPORTED; }
+
+int asc_server_destroy(asc_server* s) {
+    if (!s) return ASC_ERR_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->th.joinable()) s->th.join();
+    engine_free(s->eng[0]);
+    engine_free(s->eng[1]);
+    for (auto& kv : s->tickets) delete kv.second;
+    for (int k = 0; k < 2; k++)
+        for (SQuery* q : s->pending[k]) delete q;
+    delete s;
+    return ASC_OK;
 }
+
+int asc_submit(asc_server* s, int kind, uint32_t source, uint32_t* out, uint64_t* ticket) {
+    if (!s || !ticket || (kind != ASC_QUERY_BFS && kind != ASC_QUERY_SSSP)) return ASC_ERR_INVALID;
+    if (source >= s->g->h.V) return ASC_ERR_INVALID;
+    if (s->eng[kind].lanes == 0) {
+        set_last_error("SSSP queries need a weighted graph");
+        return ASC_ERR_UNSUPPORTED;
+    }
+    SQuery* q = new SQuery();
+    q->kind = kind;
+    q->src = source;
+    q->out = out;
+    q->t_sub = now_ms();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->error != ASC_OK) {
+            delete q;
+            return s->error;
+        }
+        if (s->outstanding >= (uint64_t)s->opts.queue_capacity) {
+            delete q;
+            return ASC_ERR_BUSY;
+        }
+        q->ticket = s->next_ticket++;
+        s->tickets[q->ticket] = q;
+        s->pending[kind].push_back(q);
+        s->submitted++;
+        s->outstanding++;
+    }
+    s->cv_work.notify_one();
+    *ticket = q->ticket;
+    return ASC_OK;
+}
+
+int asc_wait(asc_server* s, uint64_t ticket, asc_qstats* qs) {
+    if (!s) return ASC_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->tickets.find(ticket);
+    if (it == s->tickets.end()) return ASC_ERR_INVALID;
+    SQuery* q = it->second;
+    s->cv_done.wait(lk, [&] { return q->done; });
+    if (qs) {
+        qs->submit_to_admit_ms = q->t_adm - q->t_sub;
+        qs->submit_to_done_ms = q->t_done - q->t_sub;
+        qs->iterations = q->steps;
+        qs->lane = q->lane;
+    }
+    const int err = q->t_done > 0 ? ASC_OK : s->error;
+    s->tickets.erase(it);
+    delete q;
+    return err;
+}
+
+int asc_server_drain(asc_server* s) {
+    if (!s) return ASC_ERR_INVALID;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->outstanding == 0; });
+    return s->error;
+}
+
+int asc_server_get_stats(asc_server* s, asc_server_stats* st) {
+    if (!s || !st) return ASC_ERR_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    st->submitted = s->submitted;
+    st->completed = s->completed;
+    st->steps = s->steps;
+    st->lanes_bfs = s->eng[ASC_QUERY_BFS].lanes;
+    st->lanes_sssp = s->eng[ASC_QUERY_SSSP].lanes;
+    st->busy_ms = s->busy_ms;
+    return ASC_OK;
+}
+
+}  // extern "C"

@@ -311,10 +311,11 @@ int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph**
 typedef enum { ASC_QUERY_BFS = 0, ASC_QUERY_SSSP = 1 } asc_query_kind;
 
 typedef struct {
-    int lanes;                   /* max queries in flight (1..64), 0 = 64     */
-    uint64_t state_budget_bytes; /* device bytes for per-query state; 0 = auto */
+    int lanes;                   /* max queries in flight per kind (1..64), 0 = 64 */
+    uint64_t state_budget_bytes; /* device bytes for per-query state (4 V bytes
+                                    per lane); 0 = half the free memory       */
     int queue_capacity;          /* max submitted-but-unfinished queries      */
-    int keep_results_on_device;  /* 1: results are fetched with asc_fetch     */
+    int pad;
 } asc_server_opts;
 void asc_server_opts_init(asc_server_opts* o);
 
@@ -323,20 +324,30 @@ typedef struct asc_server asc_server;
 typedef struct {
     double submit_to_admit_ms;
     double submit_to_done_ms;
-    uint32_t iterations;
+    uint32_t iterations;         /* engine steps the query was in flight      */
     uint32_t lane;
 } asc_qstats;
 
+typedef struct {
+    uint64_t submitted, completed;
+    uint64_t steps;              /* engine steps run                          */
+    int lanes_bfs, lanes_sssp;   /* lanes granted by the memory budget        */
+    double busy_ms;              /* wall time with at least one query in flight */
+} asc_server_stats;
+
+/* Starts a background thread that admits and advances queries. */
 int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** out);
 int asc_server_destroy(asc_server* s);
-/* Thread-safe.  Queues a query; *ticket identifies it. */
-int asc_submit(asc_server* s, int kind, uint32_t source, uint64_t* ticket);
-/* Thread-safe.  Blocks until the query finished; copies its V values into out
- * (may be NULL to drop them) and releases the ticket. */
-int asc_wait(asc_server* s, uint64_t ticket, uint32_t* out, asc_qstats* qs);
-/* Runs the server loop until every submitted query finished (also runs in a
- * background thread started by asc_server_create). */
+/* Thread-safe.  Queues a BFS or SSSP query from `source`.  When `out` is not
+ * NULL it must hold V values; it is filled (reference convention: source 1,
+ * unreached V+1) before the query completes.  *ticket identifies the query.
+ * Returns ASC_ERR_BUSY when queue_capacity queries are outstanding. */
+int asc_submit(asc_server* s, int kind, uint32_t source, uint32_t* out, uint64_t* ticket);
+/* Thread-safe.  Blocks until the query completed and releases the ticket. */
+int asc_wait(asc_server* s, uint64_t ticket, asc_qstats* qs);
+/* Blocks until every submitted query completed. */
 int asc_server_drain(asc_server* s);
+int asc_server_get_stats(asc_server* s, asc_server_stats* st);
 
 /* ---- autotuning -------------------------------------------------------- */
 typedef struct {

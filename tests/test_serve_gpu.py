@@ -1,0 +1,119 @@
+"""GPU serving: continuous batching of BFS / SSSP queries.
+
+Each query's result in the mix must equal the same query run alone through
+the single-query engine and the oracle, bit for bit."""
+import random
+import sys
+import threading
+import time
+
+import numpy as np
+import pytest
+
+import ascetic_amd as A
+from ascetic_amd import _abi
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def wgraph(gpu_ctx):
+    h = A.rmat(15, 16, seed=11, weighted=True, wmin=1, wmax=64)
+    off, ew = h.offsets().copy(), h.edge_words().copy()
+    g = gpu_ctx.from_arrays(off, ew)
+    yield g, off, ew
+    g.close()
+
+
+def test_mix_equals_alone(wgraph):
+    g, off, ew = wgraph
+    V = len(off) - 1
+    deg = np.diff(off)
+    rng = np.random.default_rng(3)
+    cand = np.nonzero(deg > 0)[0]
+    srcs = [int(x) for x in rng.choice(cand, 150, replace=False)]
+    kinds = [A.Server.SSSP if rng.random() < 0.3 else A.Server.BFS for _ in srcs]
+    csr = O.CSR(off, ew[:, 0], ew[:, 1])
+    outs = [np.empty(V, np.uint32) for _ in srcs]
+    say("serving 150 queries")
+    with A.Server(g, lanes=64) as srv:
+        tickets = [srv.submit(k, s, o) for k, s, o in zip(kinds, srcs, outs)]
+        stats = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert st["completed"] == 150 and st["lanes_bfs"] == 64 and st["lanes_sssp"] == 64
+    assert max(s["lane"] for s in stats) > 0
+    for k, s, o in zip(kinds, srcs, outs):
+        alone, _ = (g.sssp(s) if k == A.Server.SSSP else g.bfs(s))
+        assert np.array_equal(o, alone), (k, s)
+    # spot-check against the oracle
+    for k, s, o in list(zip(kinds, srcs, outs))[:12]:
+        ref = O.dijkstra(csr, s) if k == A.Server.SSSP else O.bfs_queue(csr, s)
+        assert np.array_equal(o, ref), (k, s)
+
+
+def test_staggered_arrivals_from_threads(wgraph):
+    g, off, ew = wgraph
+    V = len(off) - 1
+    cand = np.nonzero(np.diff(off) > 0)[0]
+    results = {}
+    errors = []
+    with A.Server(g, lanes=16) as srv:
+        def client(seed):
+            r = random.Random(seed)
+            try:
+                for _ in range(12):
+                    s = int(cand[r.randrange(len(cand))])
+                    k = A.Server.SSSP if r.random() < 0.3 else A.Server.BFS
+                    out = np.empty(V, np.uint32)
+                    t = srv.submit(k, s, out)
+                    time.sleep(r.random() * 0.004)
+                    srv.wait(t)
+                    results[(seed, _)] = (k, s, out)
+            except Exception as e:  # pragma: no cover - reported below
+                errors.append(e)
+
+        threads = [threading.Thread(target=client, args=(i,)) for i in range(6)]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+    assert not errors, errors
+    assert len(results) == 72
+    for k, s, out in results.values():
+        alone, _ = (g.sssp(s) if k == A.Server.SSSP else g.bfs(s))
+        assert np.array_equal(out, alone), (k, s)
+
+
+def test_lanes_limited_by_memory_budget(wgraph):
+    g, off, ew = wgraph
+    V = len(off) - 1
+    budget = 2 * 2 * 4 * V  # two lanes per kind
+    srcs = [0, 5, 17, 99, 1000, 2047]
+    outs = [np.empty(V, np.uint32) for _ in srcs]
+    with A.Server(g, state_budget_bytes=budget) as srv:
+        assert srv.stats()["lanes_bfs"] == 2
+        ts = [srv.submit(A.Server.SSSP if i % 2 else A.Server.BFS, s, o) for i, (s, o) in enumerate(zip(srcs, outs))]
+        for t in ts:
+            srv.wait(t)
+    for i, (s, o) in enumerate(zip(srcs, outs)):
+        alone, _ = g.sssp(s) if i % 2 else g.bfs(s)
+        assert np.array_equal(o, alone)
+
+
+def test_serving_errors(gpu_ctx):
+    h = A.rmat(10, 8, seed=2)
+    with gpu_ctx.adopt(h) as g:
+        with A.Server(g, queue_capacity=2) as srv:
+            with pytest.raises(A.AscError) as e:
+                srv.submit(A.Server.SSSP, 0)
+            assert e.value.code == _abi.ERR_UNSUPPORTED
+            with pytest.raises(A.AscError) as e:
+                srv.submit(A.Server.BFS, 1 << 20)
+            assert e.value.code == _abi.ERR_INVALID
+            srv.drain()
