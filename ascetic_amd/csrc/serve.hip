@@ -208,21 +208,43 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a
 
 // -- SSSP lanes -----------------------------------------------------------------
 
-// One wavefront relaxes edge (u, v, w) for every lane in `bits`.
+// Relaxes edge (u, v, w) for every lane in `bits` (one thread per edge; the
+// lanes of one vertex share its distance row, so the loop stays within one
+// or a few cache lines of dist[v][*]).  Returns true when v must be queued.
+// Rows are `stride` (a multiple of 4) u32 long and 16-byte aligned; phase 1
+// reads both rows in 16-byte groups with no atomics in between, so the loads
+// of all active groups are in flight together; phase 2 issues atomics only
+// for lanes whose candidate beats the value read.
 template <class OffT>
-__device__ __forceinline__ void mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
-                                           unsigned long long bits, unsigned long long& act, bool& push) {
-    const uint32_t lane = lane_id();
-    bool imp = false;
-    if (lane < a.stride && ((bits >> lane) & 1)) {
-        const uint32_t cand = a.dist[(size_t)u * a.stride + lane] + wgt;
-        uint32_t* dv = a.dist + (size_t)v * a.stride + lane;
-        if (cand < *dv) imp = atomicMin(dv, cand) > cand;
+__device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
+                                           unsigned long long bits, unsigned long long& act) {
+    const uint4* du = (const uint4*)(a.dist + (size_t)u * a.stride);
+    uint32_t* dv = a.dist + (size_t)v * a.stride;
+    const uint4* dv4 = (const uint4*)dv;
+    unsigned long long maybe = 0;
+    for (unsigned long long gm = bits; gm;) {
+        const uint32_t g = ((uint32_t)__ffsll((long long)gm) - 1) >> 2;
+        gm &= ~(0xFull << (4 * g));
+        const uint4 x = du[g], y = dv4[g];
+        const uint32_t nib = (uint32_t)(bits >> (4 * g)) & 0xF;
+        uint32_t hit = 0;
+        hit |= (nib & 1) && x.x + wgt < y.x ? 1u : 0u;
+        hit |= (nib & 2) && x.y + wgt < y.y ? 2u : 0u;
+        hit |= (nib & 4) && x.z + wgt < y.z ? 4u : 0u;
+        hit |= (nib & 8) && x.w + wgt < y.w ? 8u : 0u;
+        maybe |= (unsigned long long)hit << (4 * g);
     }
-    const unsigned long long m = wave_ballot(imp);
-    act |= m;
-    push = false;
-    if (m && lane == 0) push = atomicOr(a.Y + v, m) == 0;
+    if (!maybe) return false;
+    const uint32_t* dus = (const uint32_t*)du;
+    unsigned long long imp = 0;
+    for (unsigned long long m = maybe; m; m &= m - 1) {
+        const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+        const uint32_t cand = dus[q] + wgt;
+        if (atomicMin(dv + q, cand) > cand) imp |= 1ull << q;
+    }
+    if (!imp) return false;
+    act |= imp;
+    return atomicOr(a.Y + v, imp) == 0;
 }
 
 template <class OffT>
@@ -274,20 +296,26 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a,
         s_bits[tid] = bits;
         __syncthreads();
         const uint32_t cnt = min(n - base, (uint32_t)kBlock);
-        // wave-uniform edge index: each wavefront relaxes one edge for all lanes
-        for (uint32_t e = w; e < total; e += kWavesPerBlock) {
-            uint32_t k = 0;
+        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
+            const uint32_t e = eb + lane_id();
+            bool push = false;
+            uint32_t v = 0;
+            if (e < total) {
+                uint32_t k = 0;
 #pragma unroll
-            for (uint32_t stp = kBlock / 2; stp > 0; stp >>= 1)
-                if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
-            const WEdge ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
-            bool push;
-            mssp_relax(a, s_u[k], ed.dst, ed.w, s_bits[k], act, push);
-            st.push(push, ed.dst, a.q_out, &nxt->n_res);
+                for (uint32_t stp = kBlock / 2; stp > 0; stp >>= 1)
+                    if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
+                const WEdge ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
+                v = ed.dst;
+                push = mssp_relax(a, s_u[k], v, ed.w, s_bits[k], act);
+            }
+            st.push(push, v, a.q_out, &nxt->n_res);
         }
         __syncthreads();
     }
     st.flush(a.q_out, &nxt->n_res);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
     if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
@@ -308,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a,
     const uint32_t tid = threadIdx.x, w = wave_id();
     WaveStage<kStageCap> st{s_stage[w], 0};
     unsigned long long act = 0;
-    constexpr uint32_t kSTile = 256;  // edges per workgroup tile (one per wavefront step)
+    constexpr uint32_t kSTile = kTile;
     for (uint64_t t0 = (uint64_t)blockIdx.x * kSTile; t0 < total; t0 += (uint64_t)gridDim.x * kSTile) {
         if (tid == 0) {
             uint32_t lo = 0, hi = nh - 1;
@@ -329,20 +357,27 @@ __global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a,
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)min((uint64_t)kSTile, total - t0);
-        for (uint32_t r = w; r < tlen; r += kWavesPerBlock) {
-            const uint64_t e = t0 + r;
-            uint32_t k = 0;
+        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
+            const uint32_t rr = r + lane_id();
+            bool push = false;
+            uint32_t v = 0;
+            if (rr < tlen) {
+                const uint64_t e = t0 + rr;
+                uint32_t k = 0;
 #pragma unroll
-            for (uint32_t j = 1; j < (uint32_t)kWin; j++)
-                if (j < nwin && s_hb[j] <= e) k = j;
-            const WEdge ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
-            bool push;
-            mssp_relax(a, s_hu[k], ed.dst, ed.w, s_hbits[k], act, push);
-            st.push(push, ed.dst, a.q_out, &nxt->n_res);
+                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
+                    if (j < nwin && s_hb[j] <= e) k = j;
+                const WEdge ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
+                v = ed.dst;
+                push = mssp_relax(a, s_hu[k], v, ed.w, s_hbits[k], act);
+            }
+            st.push(push, v, a.q_out, &nxt->n_res);
         }
         __syncthreads();
     }
     st.flush(a.q_out, &nxt->n_res);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
     if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
@@ -402,6 +437,7 @@ struct SQuery {
 struct Engine {
     int kind = 0;
     int lanes = 0;
+    uint32_t stride = 0;              // SSSP row length: lanes rounded up to 4
     uint64_t V = 0;
     hipStream_t stream = nullptr;
     unsigned long long* bits[2] = {nullptr, nullptr};
@@ -467,7 +503,8 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     } else {
         ASC_HIP(hipMalloc(&e.staging, 4 * V));
     }
-    hipError_t r = hipMalloc(&e.lanes_state, 4 * V * (size_t)lanes);
+    e.stride = kind == ASC_QUERY_SSSP ? (uint32_t)((lanes + 3) / 4 * 4) : (uint32_t)lanes;
+    hipError_t r = hipMalloc(&e.lanes_state, 4 * V * (size_t)e.stride);
     if (r != hipSuccess) {
         set_last_error("serving state of %d lanes x %llu vertices does not fit", lanes, (unsigned long long)V);
         return ASC_ERR_NOMEM;
@@ -501,7 +538,7 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
     a.heavy_base = e.heavy_base;
     a.s0 = e.s0;
     a.V = e.V;
-    a.stride = (uint32_t)e.lanes;
+    a.stride = e.stride;
     a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
     return a;
 }
@@ -528,7 +565,7 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
                                    e.lanes_state + (size_t)q->lane * e.V, e.V, unreached);
         } else {
             hipLaunchKernelGGL(k_reset_dist, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
-                               (uint32_t)e.lanes, mask, unreached);
+                               e.stride, mask, unreached);
         }
         hipLaunchKernelGGL((k_seed<OffT, EdgeT>), dim3(1), dim3(kMaxLanes), 0, e.stream, a, sa, e.step, e.s0,
                            e.kind == ASC_QUERY_BFS ? 1 : 0);
@@ -568,7 +605,7 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
             } else {
                 const int grid = s->g->ctx->num_cus * 4;
                 hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
-                                   (uint32_t)e.lanes, (uint32_t)q, e.staging);
+                                   e.stride, (uint32_t)q, e.staging);
                 ASC_HIP(hipMemcpyAsync(qq->out, e.staging, 4 * e.V, hipMemcpyDeviceToHost, e.stream));
             }
             ASC_HIP(hipStreamSynchronize(e.stream));

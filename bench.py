@@ -52,6 +52,84 @@ def algorithmic_bytes(kern, s_e, s_o):
     }
 
 
+def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0):
+    """Query serving on a resident weighted R-MAT graph: 70% BFS / 30% SSSP,
+    sources uniform over vertices with degree > 0.  Reports one-query-per-
+    launch throughput (plain kernels), the server's saturated throughput and
+    Poisson-arrival sweeps with p50/p99 submit-to-done latency, and checks a
+    sample of served results against the same queries run alone."""
+    import threading
+
+    t = time.perf_counter()
+    h = A.rmat(scale, 16, seed=seed, weighted=True, wmin=1, wmax=64)
+    V = h.num_vertices
+    deg = np.diff(h.offsets())
+    cand = np.nonzero(deg > 0)[0]
+    log(f"serving graph: R-MAT s{scale} ef16 weighted, V={V} E={h.num_edges} ({time.perf_counter() - t:.1f}s)")
+    rng = np.random.default_rng(seed)
+    srcs = [int(x) for x in rng.choice(cand, n_queries)]
+    kinds = [A.Server.SSSP if x < 0.3 else A.Server.BFS for x in rng.random(n_queries)]
+    out = {"workload": f"rmat_s{scale}_ef16_w1-64_resident_70bfs_30sssp", "queries": n_queries}
+    with A.Context() as ctx:
+        g = ctx.adopt(h, offset_bytes=8)
+        # one query per launch
+        for k, s in zip(kinds[:2], srcs[:2]):
+            (g.sssp if k == A.Server.SSSP else g.bfs)(s, output=False)
+        n_plain = min(n_queries, 64)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for k, s in zip(kinds[:n_plain], srcs[:n_plain]):
+            (g.sssp if k == A.Server.SSSP else g.bfs)(s, output=False)
+        ctx.synchronize()
+        plain_qps = n_plain / (time.perf_counter() - t0)
+        out["plain_qps"] = round(plain_qps, 2)
+        with A.Server(g, lanes=64) as srv:
+            st = srv.stats()
+            out["lanes"] = {"bfs": st["lanes_bfs"], "sssp": st["lanes_sssp"]}
+            # exactness on a sample, served in a full mix
+            sample = list(range(0, n_queries, max(1, n_queries // 8)))[:8]
+            bufs = {i: np.empty(V, np.uint32) for i in sample}
+            t0 = time.perf_counter()
+            tickets = [srv.submit(k, s, bufs.get(i)) for i, (k, s) in enumerate(zip(kinds, srcs))]
+            lat = [srv.wait(tk)["submit_to_done_ms"] for tk in tickets]
+            sat = n_queries / (time.perf_counter() - t0)
+            exact = all(np.array_equal(bufs[i], (g.sssp if kinds[i] == A.Server.SSSP else g.bfs)(srcs[i])[0])
+                        for i in sample)
+            out["saturated_qps"] = round(sat, 2)
+            out["saturated_p50_ms"] = round(float(np.percentile(lat, 50)), 3)
+            out["saturated_p99_ms"] = round(float(np.percentile(lat, 99)), 3)
+            out["speedup_vs_plain"] = round(sat / plain_qps, 2)
+            out["exact_sample"] = f"{len(sample)} queries bit-identical to running alone" if exact else "MISMATCH"
+            sweeps = []
+            for f in rate_factors:
+                rate = f * plain_qps
+                n = min(n_queries, max(16, int(rate * min(budget_s / len(rate_factors), 10.0))))
+                gaps = np.random.default_rng(seed + int(f * 100)).exponential(1.0 / rate, n)
+                tks = []
+
+                def submitter():
+                    nxt = time.perf_counter()
+                    for i in range(n):
+                        nxt += gaps[i]
+                        d = nxt - time.perf_counter()
+                        if d > 0:
+                            time.sleep(d)
+                        tks.append(srv.submit(kinds[i], srcs[i]))
+
+                ts = time.perf_counter()
+                th = threading.Thread(target=submitter)
+                th.start()
+                th.join()
+                lats = [srv.wait(tk)["submit_to_done_ms"] for tk in tks]
+                el = time.perf_counter() - ts
+                sweeps.append({"offered_qps": round(rate, 2), "achieved_qps": round(n / el, 2), "queries": n,
+                               "p50_ms": round(float(np.percentile(lats, 50)), 3),
+                               "p99_ms": round(float(np.percentile(lats, 99)), 3)})
+            out["poisson"] = sweeps
+        g.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +144,8 @@ def main():
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving section (0 = skip)")
+    ap.add_argument("--serve-queries", type=int, default=512)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -184,9 +264,12 @@ def main():
         result["parity"] = "bit-exact vs oracle" if ok else "MISMATCH vs oracle"
         if not ok:
             log("GPU distances differ from the oracle")
-    print(json.dumps(result), flush=True)
     g.close()
     ctx.close()
+    del h, off, ew
+    if args.serve_scale > 0:
+        result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0])
+    print(json.dumps(result), flush=True)
     if result.get("parity", "").startswith("MISMATCH"):
         sys.exit(1)
 
