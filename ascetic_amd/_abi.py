@@ -49,6 +49,7 @@ class Config(C.Structure):
         ("sssp_delta", C.c_int),
         ("verbose", C.c_int),
         ("profile", C.c_int),
+        ("stream_pack", C.c_int),
     ]
 
 

@@ -28,6 +28,8 @@ void graph_free_device(asc_graph* g) {
     if (g->pr) pr_free(g->pr);
     g->pr = nullptr;
     g->zc_tail.release();
+    g->ptail.release();
+    g->packed = false;
 }
 
 static int param_size_for(int kind) {
@@ -89,8 +91,42 @@ int graph_upload(asc_graph* g) {
         ASC_HIP(hipMemcpy(g->d_outdeg, g->h.outdeg.p, 4 * V, hipMemcpyHostToDevice));
         g->device_bytes += 4 * V;
     }
+    // packed on-demand stream: one u32 per edge when the id and weight fit
+    g->packed = false;
+    if (ctx->cfg.stream_pack && g->h.kind == ASC_KIND_WCSR && g->static_edges < E) {
+        const uint64_t S = g->static_edges, n = E - S;
+        const uint32_t* ew = g->h.edges.p;
+        std::mutex mu;
+        uint32_t wlo = 0xFFFFFFFFu, whi = 0;
+        host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
+            uint32_t lo = 0xFFFFFFFFu, hi = 0;
+            for (uint64_t i = b; i < en; i++) {
+                const uint32_t w = ew[2 * (S + i) + 1];
+                lo = std::min(lo, w);
+                hi = std::max(hi, w);
+            }
+            std::lock_guard<std::mutex> lk(mu);
+            wlo = std::min(wlo, lo);
+            whi = std::max(whi, hi);
+        });
+        const uint32_t dbits = V > 1 ? 64 - (uint32_t)__builtin_clzll(V - 1) : 1;
+        const uint32_t span = whi - wlo;
+        const uint32_t wbits = span ? 32 - (uint32_t)__builtin_clz(span) : 0;
+        if (dbits + wbits <= 32 && dbits < 32) {
+            ASC_TRY(g->ptail.alloc(n, true));
+            host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
+                for (uint64_t i = b; i < en; i++)
+                    g->ptail[i] = ew[2 * (S + i)] | ((ew[2 * (S + i) + 1] - wlo) << dbits);
+            });
+            g->packed = true;
+            g->pk_shift = dbits;
+            g->pk_wmin = wlo;
+            const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(256) << 20);
+            g->part_edges = std::max<uint64_t>(pb / 4, 1024);
+        }
+    }
     // zero-copy streaming reads the on-demand edges from mapped pinned memory
-    if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY && g->static_edges < E) {
+    if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY && g->static_edges < E && !g->packed) {
         const uint64_t words = (E - g->static_edges) * g->h.edge_words();
         ASC_TRY(g->zc_tail.alloc(words, true));
         const uint32_t* src = g->h.edges.p + g->static_edges * g->h.edge_words();

@@ -127,6 +127,8 @@ void asc_config_init(asc_config* c) {
     c->pr_group = 0;
     c->sssp_delta = 0;
     c->verbose = 0;
+    c->profile = 0;
+    c->stream_pack = 1;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

@@ -142,9 +142,11 @@ int hardware_threads();
 ThreadPool& host_pool();
 
 // Copies edges [lo, hi) of the concatenated adjacency of list[0..m)
-// (list[k] owns [prefix[k], prefix[k+1])) from the host edge array into dst.
-void gather_edges(ThreadPool& pool, const char* edges, const uint64_t* off, size_t EB, const uint32_t* list,
-                  const unsigned long long* prefix, uint64_t m, uint64_t lo, uint64_t hi, char* dst);
+// (list[k] owns [prefix[k], prefix[k+1])) into dst.  `edges` holds records
+// of EB bytes for edge indices starting at `base` (edge i at edges[i - base]).
+void gather_edges(ThreadPool& pool, const char* edges, uint64_t base, const uint64_t* off, size_t EB,
+                  const uint32_t* list, const unsigned long long* prefix, uint64_t m, uint64_t lo, uint64_t hi,
+                  char* dst);
 
 }  // namespace asc
 
@@ -180,6 +182,12 @@ struct asc_graph {
     uint64_t device_bytes = 0;
     // pinned mirror of the on-demand edges for zero-copy streaming
     asc::HostBuf<uint32_t> zc_tail;
+    // On-demand edges packed one per u32 word, dst | (w - wmin) << pk_shift,
+    // when the id and weight bits fit (weighted graphs only); pinned.  The
+    // streamed parts then carry 4 instead of 8 bytes per edge.
+    asc::HostBuf<uint32_t> ptail;
+    bool packed = false;
+    uint32_t pk_shift = 32, pk_wmin = 0;
     // tunables
     int heavy_threshold = 0;
     int pr_group = 0;

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
 // entries; sv[k] is the value list[k] is expanded with).  ZC = false: edges
 // come from the device part buffer holding [lo, hi).  ZC = true: from mapped
 // pinned host memory (tail = edges from static_edges on).
-template <int ALGO, class OffT, class EdgeT, bool ZC>
+template <int ALGO, class OffT, class EdgeT, bool ZC, bool PK>
 __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p,
                                                          const uint32_t* list_sv) {
     __shared__ unsigned long long s_p[kTile];
@@ -280,11 +280,19 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
 #pragma unroll
                 for (uint32_t step = kTile / 2; step > 0; step >>= 1)
                     if (k + step < m && s_p[k + step] <= e) k += step;
-                EdgeT ed;
-                if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
-                else ed = p.part[e - p.lo];
-                v = edge_dst(ed);
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+                uint32_t w;
+                if (PK) {
+                    const uint32_t x = ZC ? p.zc32[s_src[k] + (e - s_p[k])] : p.part32[e - p.lo];
+                    v = x & p.pk_mask;
+                    w = (x >> p.pk_shift) + p.pk_wmin;
+                } else {
+                    EdgeT ed;
+                    if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
+                    else ed = p.part[e - p.lo];
+                    v = edge_dst(ed);
+                    w = edge_w(ed);
+                }
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], w), level);
             }
             pq.push(a, nxt, to, v);
         }
@@ -305,15 +313,17 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
 template <int ALGO, class OffT>
 __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const uint32_t* pile, uint32_t n,
                                                         uint32_t* val, uint32_t* done, uint32_t* out,
-                                                        uint32_t* out_sv, uint32_t* count) {
+                                                        uint32_t* out_sv, uint32_t* out_deg, uint32_t* count) {
     const uint32_t lane = lane_id();
     for (uint32_t base = (blockIdx.x * kBlock) & ~(kWave - 1u); base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
         bool keep = false;
-        uint32_t v = 0, d = 0;
+        uint32_t v = 0, d = 0, deg = 0;
         if (i < n) {
             v = pile[i];
-            keep = off[v + 1] > off[v];
+            const uint64_t dd = (uint64_t)(off[v + 1] - off[v]);
+            deg = dd > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dd;
+            keep = dd > 0;
             if (ALGO != ALG_BFS && keep) {
                 d = val[v];
                 keep = done[v] != d;
@@ -328,6 +338,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const u
         if (keep) {
             const uint32_t pos = at + lanes_below(m);
             out[pos] = v;
+            out_deg[pos] = deg;
             if (ALGO != ALG_BFS) out_sv[pos] = d;
         }
     }
@@ -437,7 +448,7 @@ void trav_free(TravState* t) {
 
 static int trav_ensure(asc_graph* g) {
     const uint64_t V = g->h.V;
-    const size_t EB = g->h.edge_bytes();
+    const size_t EB = g->packed ? 4 : g->h.edge_bytes();  // bytes per streamed edge
     TravState* t = g->trav;
     const bool need_parts = g->static_vertices < V;
     if (t && t->V == V && (!need_parts || (t->od_list && t->part_edges >= g->part_edges))) return ASC_OK;
@@ -463,10 +474,11 @@ static int trav_ensure(asc_graph* g) {
         const uint64_t nod = V - g->static_vertices;
         ASC_HIP(hipMalloc(&t->pend[0], 4 * nod));
         ASC_HIP(hipMalloc(&t->pend[1], 4 * nod));
-        // od_list holds the filtered list and, after it, the snapshot values
-        ASC_HIP(hipMalloc(&t->od_list, 8 * nod));
+        // od_list holds the filtered list, then the snapshot values, then the
+        // degrees (nod entries each)
+        ASC_HIP(hipMalloc(&t->od_list, 12 * nod));
         ASC_HIP(hipMalloc(&t->od_prefix, 8 * (nod + 1)));
-        ASC_TRY(t->h_list.alloc(nod, true));
+        ASC_TRY(t->h_list.alloc(2 * nod, true));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
         for (int i = 0; i < 2; i++) {
@@ -545,8 +557,12 @@ static uint32_t sssp_delta(asc_graph* g) {
 // fillEdgeArrByMultiThread, GraphMeta.cuh:469-514, cut to one part).
 static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long long* prefix, uint64_t m,
                         uint64_t lo, uint64_t hi, char* dst) {
-    gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, g->h.off.p, g->h.edge_bytes(), list, prefix, m, lo, hi,
-                 dst);
+    if (g->packed)
+        gather_edges(*g->ctx->pool, (const char*)g->ptail.p, g->static_edges, g->h.off.p, 4, list, prefix, m, lo, hi,
+                     dst);
+    else
+        gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, 0, g->h.off.p, g->h.edge_bytes(), list, prefix, m, lo,
+                     hi, dst);
 }
 
 // Streams one pile: filter on the device, list to the host, u64 prefix of
@@ -561,9 +577,11 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     const double t_begin = now_ms();
     const uint64_t nod = g->h.V - g->static_vertices;
     uint32_t* list_sv = t->od_list + nod;
+    uint32_t* list_deg = t->od_list + 2 * nod;
     hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
     hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096)),
-                       dim3(kBlock), 0, s, a.off, pile, npile, a.val, a.done, t->od_list, list_sv, &t->piles->n_list);
+                       dim3(kBlock), 0, s, a.off, pile, npile, a.val, a.done, t->od_list, list_sv, list_deg,
+                       &t->piles->n_list);
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
     ASC_HIP(hipStreamSynchronize(s));
@@ -572,16 +590,18 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
         rc.pre_ms += now_ms() - t_begin;
         return ASC_OK;
     }
+    // the kept list and its degrees (computed on the device)
+    uint32_t* h_deg = t->h_list.p + nod;
     ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipMemcpyAsync(h_deg, list_deg, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
     ASC_HIP(hipStreamSynchronize(s));
-    // u64 prefix of the degrees, in parallel
-    const uint64_t* off = g->h.off.p;
+    // u64 exclusive prefix of the degrees, in parallel
     const int nt = ctx->pool->size();
     std::vector<uint64_t> esum(nt + 1, 0);
     const uint64_t chunk = (m + nt - 1) / nt;
     ctx->pool->run([&](int tt, int) {
         uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = 0;
-        for (uint64_t i = b; i < e; i++) acc += off[t->h_list[i] + 1] - off[t->h_list[i]];
+        for (uint64_t i = b; i < e; i++) acc += h_deg[i];
         esum[tt + 1] = acc;
     });
     for (int i = 0; i < nt; i++) esum[i + 1] += esum[i];
@@ -589,7 +609,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
         uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = esum[tt];
         for (uint64_t i = b; i < e; i++) {
             t->h_prefix[i] = acc;
-            acc += off[t->h_list[i] + 1] - off[t->h_list[i]];
+            acc += h_deg[i];
         }
     });
     const uint64_t T = esum[nt];
@@ -604,17 +624,35 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     p.prefix = t->od_prefix;
     p.n = m;
     p.tail_base = g->static_edges;
-    const size_t EB = g->h.edge_bytes();
+    p.pk_shift = g->pk_shift;
+    p.pk_mask = g->packed ? (1u << g->pk_shift) - 1 : 0xFFFFFFFFu;
+    p.pk_wmin = g->pk_wmin;
+    const bool pk = g->packed;
+    const size_t EB = pk ? 4 : g->h.edge_bytes();
+    auto launch_part = [&](bool zc) {
+        return timed_launch(t, prof, KERN_PART, s, [&] {
+            if (zc && pk)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, true>), dim3(grid), dim3(kBlock), 0, s, a,
+                                   level, p, list_sv);
+            else if (zc)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, false>), dim3(grid), dim3(kBlock), 0, s, a,
+                                   level, p, list_sv);
+            else if (pk)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, true>), dim3(grid), dim3(kBlock), 0, s,
+                                   a, level, p, list_sv);
+            else
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, false>), dim3(grid), dim3(kBlock), 0, s,
+                                   a, level, p, list_sv);
+        });
+    };
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
         void* dptr = nullptr;
-        ASC_HIP(hipHostGetDevicePointer(&dptr, g->zc_tail.p, 0));
+        ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
         p.zc = (const EdgeT*)dptr;
+        p.zc32 = (const uint32_t*)dptr;
         p.lo = 0;
         p.hi = T;
-        ASC_TRY(timed_launch(t, prof, KERN_PART, s, [&] {
-            hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true>), dim3(grid), dim3(kBlock), 0, s, a, level, p,
-                               list_sv);
-        }));
+        ASC_TRY(launch_part(true));
         rc.parts += 1;
     } else {
         const uint64_t PE = t->part_edges;
@@ -631,12 +669,10 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.part = (const EdgeT*)t->part[b];
+            p.part32 = (const uint32_t*)t->part[b];
             p.lo = lo;
             p.hi = hi;
-            ASC_TRY(timed_launch(t, prof, KERN_PART, s, [&] {
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false>), dim3(grid), dim3(kBlock), 0, s, a,
-                                   level, p, list_sv);
-            }));
+            ASC_TRY(launch_part(false));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
         }
         rc.parts += (uint32_t)np;
@@ -841,7 +877,7 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         st->static_edges_traversed = ds.edges_local + ds.edges_heavy;
         st->edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_part;
         st->streamed_edges = rc.streamed_edges;
-        st->streamed_bytes = rc.streamed_edges * g->h.edge_bytes();
+        st->streamed_bytes = rc.streamed_edges * (g->packed ? 4 : g->h.edge_bytes());
         st->vertices_processed = ds.verts_local;
         st->host_cores = hardware_threads();
         std::memcpy(st->kern, kern, sizeof(kern));

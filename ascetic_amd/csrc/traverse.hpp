@@ -81,7 +81,10 @@ struct PartArgs {
     unsigned long long lo, hi;      // edge range of this part
     const EdgeT* part;              // device buffer holding edges [lo, hi)
     const EdgeT* zc;                // zero-copy: mapped host edges from tail_base
+    const uint32_t* part32;         // packed stream: device buffer of u32 words
+    const uint32_t* zc32;           // packed stream, zero-copy
     unsigned long long tail_base;
+    uint32_t pk_shift, pk_mask, pk_wmin;  // word = dst | (w - pk_wmin) << pk_shift
 };
 
 struct RunCounters {

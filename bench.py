@@ -241,6 +241,7 @@ def main():
             "streamed_bytes_per_step": streamed,
             "achieved_GBps": round(streamed / step_s / 1e9, 3),
             "gather_ms_per_step": sum(r["fill_ms"] for r in runs) / args.steps,
+            "prepare_ms_per_step": sum(r["pre_ms"] for r in runs) / args.steps,
             "overload_ms_per_step": sum(r["overload_ms"] for r in runs) / args.steps,
         },
     }

@@ -114,6 +114,9 @@ typedef struct {
     int verbose;                 /* print per-phase timings like the reference */
     int profile;                 /* 1: time every relax kernel with HIP events
                                     on its launch stream (asc_stats.kern)    */
+    int stream_pack;             /* 1 (default): stream weighted on-demand
+                                    edges as one u32 word, dst | (w - wmin) <<
+                                    bits(V-1), when both fit; 0: 8-byte records */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;

@@ -91,11 +91,17 @@ def test_streaming_matches_resident(name, mk, mode):
             lab, st = g.cc()
             assert np.array_equal(lab, O.cc_minreach(csr))
             assert st["streamed_edges"] > 0
-    with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_mode=mode) as ctx:
-        with make_graph(ctx, off, dst, w) as g:
-            for s in (0, int(np.argmax(np.diff(off)))):
-                d, _ = g.sssp(s)
-                assert np.array_equal(d, O.dijkstra(csr, s))
+    for pack in (True, False):  # 4-byte packed stream and 8-byte {dst, w} records
+        with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_mode=mode,
+                       stream_pack=pack) as ctx:
+            with make_graph(ctx, off, dst, w) as g:
+                for s in (0, int(np.argmax(np.diff(off)))):
+                    d, st = g.sssp(s)
+                    assert np.array_equal(d, O.dijkstra(csr, s))
+                    if st["streamed_edges"]:
+                        assert st["streamed_bytes"] == st["streamed_edges"] * (4 if pack else 8)
+                    lv, _ = g.bfs(s)
+                    assert np.array_equal(lv, O.bfs_queue(csr, s))
 
 
 def test_everything_on_demand(gpu_ctx):
