@@ -213,6 +213,12 @@ class Context:
     def synchronize(self):
         check(_abi.load().asc_ctx_synchronize(self._h), "asc_ctx_synchronize")
 
+    def measure_link(self, nbytes=256 << 20, reps=8):
+        """Pinned host <-> device copy bandwidth in GB/s: (h2d, d2h)."""
+        h2d, d2h = C.c_double(), C.c_double()
+        check(_abi.load().asc_measure_link(self._h, nbytes, reps, C.byref(h2d), C.byref(d2h)), "asc_measure_link")
+        return h2d.value, d2h.value
+
     def load(self, path, fmt=_abi.FMT_AUTO):
         h = C.c_void_p()
         check(_abi.load().asc_graph_load(self._h, path.encode(), fmt, C.byref(h)), f"asc_graph_load({path})")

@@ -121,7 +121,7 @@ int graph_upload(asc_graph* g) {
             g->packed = true;
             g->pk_shift = dbits;
             g->pk_wmin = wlo;
-            const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(256) << 20);
+            const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
             g->part_edges = std::max<uint64_t>(pb / 4, 1024);
         }
     }
@@ -225,6 +225,40 @@ int asc_ctx_synchronize(asc_ctx* ctx) {
     ASC_HIP(hipSetDevice(ctx->device));
     ASC_HIP(hipDeviceSynchronize());
     return ASC_OK;
+}
+
+int asc_measure_link(asc_ctx* ctx, uint64_t bytes, int reps, double* h2d_gbps, double* d2h_gbps) {
+    if (!ctx || bytes == 0 || reps <= 0) return ASC_ERR_INVALID;
+    ASC_HIP(hipSetDevice(ctx->device));
+    HostBuf<char> host;
+    ASC_TRY(host.alloc(bytes, true));
+    std::memset(host.p, 1, bytes);
+    void* dev = nullptr;
+    if (hipMalloc(&dev, bytes) != hipSuccess) return ASC_ERR_NOMEM;
+    hipEvent_t a, b;
+    int rc = ASC_OK;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = ASC_ERR_DEVICE;
+    for (int dir = 0; dir < 2 && rc == ASC_OK; dir++) {
+        auto copy = [&] {
+            return dir == 0 ? hipMemcpyAsync(dev, host.p, bytes, hipMemcpyHostToDevice, ctx->s_copy)
+                            : hipMemcpyAsync(host.p, dev, bytes, hipMemcpyDeviceToHost, ctx->s_copy);
+        };
+        if (copy() != hipSuccess) rc = ASC_ERR_DEVICE;  // warm-up
+        (void)hipEventRecord(a, ctx->s_copy);
+        for (int i = 0; i < reps && rc == ASC_OK; i++)
+            if (copy() != hipSuccess) rc = ASC_ERR_DEVICE;
+        (void)hipEventRecord(b, ctx->s_copy);
+        (void)hipEventSynchronize(b);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, a, b);
+        const double gbps = ms > 0 ? (double)bytes * reps / (ms * 1e-3) / 1e9 : 0;
+        if (dir == 0 && h2d_gbps) *h2d_gbps = gbps;
+        if (dir == 1 && d2h_gbps) *d2h_gbps = gbps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(dev);
+    return rc;
 }
 
 int asc_graph_load(asc_ctx* ctx, const char* path, int format, asc_graph** out) {

@@ -60,7 +60,7 @@ extern "C" int asc_plan_partition(uint64_t V, uint64_t E, const uint64_t* off, u
     out->static_edges = off[vs];
     out->static_vertices = vs;
     out->ondemand_edges = E - off[vs];
-    const uint64_t pb = part_bytes ? part_bytes : (uint64_t(256) << 20);
+    const uint64_t pb = part_bytes ? part_bytes : (uint64_t(64) << 20);
     out->part_edges = std::max<uint64_t>(pb / edge_size, 1024);
     return ASC_OK;
 }

@@ -583,18 +583,18 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
                        dim3(kBlock), 0, s, a.off, pile, npile, a.val, a.done, t->od_list, list_sv, list_deg,
                        &t->piles->n_list);
     ASC_HIP(hipGetLastError());
+    // count, kept list and degrees (computed on the device) in one round trip:
+    // the kept count is at most the pile size, so copy that many entries
+    uint32_t* h_deg = t->h_list.p + nod;
     ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipMemcpyAsync(h_deg, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
     ASC_HIP(hipStreamSynchronize(s));
     const uint32_t m = t->h_piles[0].n_list;
     if (m == 0) {
         rc.pre_ms += now_ms() - t_begin;
         return ASC_OK;
     }
-    // the kept list and its degrees (computed on the device)
-    uint32_t* h_deg = t->h_list.p + nod;
-    ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
-    ASC_HIP(hipMemcpyAsync(h_deg, list_deg, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
-    ASC_HIP(hipStreamSynchronize(s));
     // u64 exclusive prefix of the degrees, in parallel
     const int nt = ctx->pool->size();
     std::vector<uint64_t> esum(nt + 1, 0);
@@ -630,7 +630,9 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     const bool pk = g->packed;
     const size_t EB = pk ? 4 : g->h.edge_bytes();
     auto launch_part = [&](bool zc) {
+        const int pgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (p.hi - p.lo + kTile - 1) / kTile));
         return timed_launch(t, prof, KERN_PART, s, [&] {
+            const int grid = pgrid;
             if (zc && pk)
                 hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, true>), dim3(grid), dim3(kBlock), 0, s, a,
                                    level, p, list_sv);
@@ -813,9 +815,10 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
                 if (!e0) return ASC_ERR_DEVICE;
                 ASC_HIP(hipEventRecord(e0, s));
             }
+            // one workgroup per 256 frontier entries, at most `grid`
+            const int lgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, ((uint64_t)nr + kBlock - 1) / kBlock));
             ASC_TRY(timed_launch(t, prof, KERN_LOCAL, s, [&] {
-                hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(nr ? grid : 1), dim3(kBlock), 0, s, a,
-                                   level);
+                hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(lgrid), dim3(kBlock), 0, s, a, level);
             }));
             if (nr) {
                 ASC_TRY(timed_launch(t, prof, KERN_HEAVY, s, [&] {

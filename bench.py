@@ -139,7 +139,7 @@ def main():
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=24)
     ap.add_argument("--budget-frac", type=float, default=0.4)
-    ap.add_argument("--part-mb", type=int, default=256)
+    ap.add_argument("--part-mb", type=int, default=64)
     ap.add_argument("--stream-mode", choices=["gather", "zerocopy"], default="gather")
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -198,6 +198,7 @@ def main():
     dk = kern[dom]
     achieved = abytes[dom] / (dk["ms"] / 1e3) / 1e9 if dk["ms"] > 0 else 0.0
     streamed = sum(r["streamed_bytes"] for r in runs) / args.steps
+    link_h2d, link_d2h = ctx.measure_link(256 << 20, 8)
 
     result = {
         "metric": METRIC,
@@ -240,6 +241,11 @@ def main():
         "link": {
             "streamed_bytes_per_step": streamed,
             "achieved_GBps": round(streamed / step_s / 1e9, 3),
+            "peak_h2d_GBps": round(link_h2d, 2),
+            "peak_d2h_GBps": round(link_d2h, 2),
+            "frac": round(streamed / step_s / 1e9 / link_h2d, 4) if link_h2d else None,
+            "stream_record_bytes": 4 if int(streamed) == 4 * int(sum(r["streamed_edges"] for r in runs) / args.steps)
+            else 8,
             "gather_ms_per_step": sum(r["fill_ms"] for r in runs) / args.steps,
             "prepare_ms_per_step": sum(r["pre_ms"] for r in runs) / args.steps,
             "overload_ms_per_step": sum(r["overload_ms"] for r in runs) / args.steps,

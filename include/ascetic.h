@@ -129,6 +129,10 @@ int asc_ctx_destroy(asc_ctx* ctx);
 int asc_device_count(void);
 /* Blocks until all work the library queued on the context's device ended. */
 int asc_ctx_synchronize(asc_ctx* ctx);
+/* Host-link probe: copies `bytes` between a pinned host buffer and the
+ * device `reps` times in each direction (HIP events on the copy stream) and
+ * reports GB/s; the roofline for streamed on-demand edges. */
+int asc_measure_link(asc_ctx* ctx, uint64_t bytes, int reps, double* h2d_gbps, double* d2h_gbps);
 
 /* ---- graphs ------------------------------------------------------------ */
 /* Host view of a CSR/CSC graph.  Exactly one of off64/off32 is set; it holds
