@@ -53,11 +53,6 @@ int graph_upload(asc_graph* g) {
     g->static_vertices = plan.static_vertices;
     g->part_edges = plan.part_edges;
     g->advise_k = plan.advise_k;
-    if (g->h.kind == ASC_KIND_CSC && g->static_vertices < V) {
-        // PageRank pulls over every vertex each iteration; keep it resident.
-        g->static_edges = E;
-        g->static_vertices = V;
-    }
 
     // offsets
     const size_t ob = g->off_bytes;
@@ -125,8 +120,10 @@ int graph_upload(asc_graph* g) {
             g->part_edges = std::max<uint64_t>(pb / 4, 1024);
         }
     }
-    // zero-copy streaming reads the on-demand edges from mapped pinned memory
-    if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY && g->static_edges < E && !g->packed) {
+    // zero-copy streaming reads the on-demand edges from mapped pinned memory;
+    // PageRank streams whole vertex-range chunks of them by DMA every iteration
+    const bool pr_tail = g->h.kind == ASC_KIND_CSC;
+    if ((ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || pr_tail) && g->static_edges < E && !g->packed) {
         const uint64_t words = (E - g->static_edges) * g->h.edge_words();
         ASC_TRY(g->zc_tail.alloc(words, true));
         const uint32_t* src = g->h.edges.p + g->static_edges * g->h.edge_words();

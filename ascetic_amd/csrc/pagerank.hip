@@ -38,6 +38,16 @@ struct PrState {
     uint8_t* active = nullptr;
     uint32_t* count = nullptr;
     HostBuf<uint32_t> h_count;
+    // on-demand in-edges: vertex-range chunks [v0, v1) whose in-edges
+    // [off[v0], off[v1]) fit one part buffer, copied from the pinned tail
+    struct Chunk {
+        uint64_t v0, v1, e0, e1;
+    };
+    std::vector<Chunk> chunks;
+    uint32_t* part[2] = {nullptr, nullptr};
+    uint64_t part_edges = 0;
+    hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr};
+    uint64_t streamed_edges = 0;  // per pull sweep
 };
 
 void pr_free(PrState* p) {
@@ -47,7 +57,62 @@ void pr_free(PrState* p) {
     };
     f(p->rank[0]); f(p->rank[1]); f(p->contrib[0]); f(p->contrib[1]);
     f(p->acc); f(p->heavy); f(p->val64); f(p->sum64); f(p->contrib64); f(p->active); f(p->count);
+    f(p->part[0]); f(p->part[1]);
+    for (int i = 0; i < 2; i++) {
+        if (p->ev_copy[i]) (void)hipEventDestroy(p->ev_copy[i]);
+        if (p->ev_kern[i]) (void)hipEventDestroy(p->ev_kern[i]);
+    }
     delete p;
+}
+
+// Plans the on-demand chunks of a partially resident PageRank graph (once).
+static int pr_plan_chunks(asc_graph* g, PrState* p) {
+    const uint64_t V = g->h.V, vs = g->static_vertices;
+    if (vs >= V || !p->chunks.empty()) return ASC_OK;
+    const uint64_t* off = g->h.off.p;
+    uint64_t maxdeg = 0;
+    for (uint64_t v = vs; v < V; v++) maxdeg = std::max<uint64_t>(maxdeg, off[v + 1] - off[v]);
+    // a part holds at least the largest on-demand in-list
+    p->part_edges = std::max<uint64_t>(g->part_edges, maxdeg);
+    uint64_t v = vs;
+    while (v < V) {
+        uint64_t v1 = v;
+        while (v1 < V && off[v1 + 1] - off[v] <= p->part_edges) v1++;
+        if (v1 == v) v1 = v + 1;
+        p->chunks.push_back({v, v1, off[v], off[v1]});
+        v = v1;
+    }
+    p->streamed_edges = off[V] - off[vs];
+    for (int i = 0; i < 2; i++) {
+        ASC_HIP(hipMalloc(&p->part[i], 4 * std::max<uint64_t>(p->part_edges, 1)));
+        ASC_HIP(hipEventCreateWithFlags(&p->ev_copy[i], hipEventDisableTiming));
+        ASC_HIP(hipEventCreateWithFlags(&p->ev_kern[i], hipEventDisableTiming));
+        // recorded once so that every copy can wait for the buffer's last reader
+        ASC_HIP(hipEventRecord(p->ev_kern[i], g->ctx->s_main));
+    }
+    return ASC_OK;
+}
+
+// Enqueues, for every on-demand chunk, its copy from the pinned tail on the
+// copy stream (double-buffered) and `pull(src, e_base, v0, v1)` on the compute
+// stream.  No host synchronisation: every copy waits for the previous kernel
+// that read its buffer, also across iterations.
+template <class F>
+static int pr_stream_chunks(asc_graph* g, PrState* p, F&& pull) {
+    asc_ctx* ctx = g->ctx;
+    const uint64_t S = g->static_edges;
+    for (size_t c = 0; c < p->chunks.size(); c++) {
+        const auto& ch = p->chunks[c];
+        const int b = (int)(c & 1);
+        ASC_HIP(hipStreamWaitEvent(ctx->s_copy, p->ev_kern[b], 0));
+        ASC_HIP(hipMemcpyAsync(p->part[b], g->zc_tail.p + (ch.e0 - S), 4 * (ch.e1 - ch.e0), hipMemcpyHostToDevice,
+                               ctx->s_copy));
+        ASC_HIP(hipEventRecord(p->ev_copy[b], ctx->s_copy));
+        ASC_HIP(hipStreamWaitEvent(ctx->s_main, p->ev_copy[b], 0));
+        ASC_TRY(pull(p->part[b], ch.e0, ch.v0, ch.v1));
+        ASC_HIP(hipEventRecord(p->ev_kern[b], ctx->s_main));
+    }
+    return ASC_OK;
 }
 
 // ---- normalised mode --------------------------------------------------------
@@ -74,7 +139,10 @@ __global__ void k_pr_init(float* rank, float* contrib, const uint32_t* outdeg, u
 
 struct PrArgs {
     const void* off;              // OffT[V+1]
-    const uint32_t* src;          // in-edges
+    const uint32_t* src;          // in-edges: edge k at src[k - e_base]
+    unsigned long long e_base;
+    unsigned long long v_begin, v_end;  // vertices pulled by this launch
+    int skip_heavy;               // leave in-degree >= heavy_thr to k_pr_pull_heavy
     const uint32_t* outdeg;
     const float* rank_in;
     const float* contrib_in;
@@ -126,13 +194,15 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
     const double base = (1.0 - a.damping64) / (double)a.V + a.damping64 * (*a.dang_in) / (double)a.V;
     const uint32_t sub = threadIdx.x % G;
     const uint64_t groups_per_grid = (uint64_t)gridDim.x * (blockDim.x / G);
+    const uint32_t* src = a.src - a.e_base;
     double res = 0, dang = 0;
-    for (uint64_t v = blockIdx.x * (uint64_t)(blockDim.x / G) + threadIdx.x / G; v < a.V; v += groups_per_grid) {
+    for (uint64_t v = a.v_begin + blockIdx.x * (uint64_t)(blockDim.x / G) + threadIdx.x / G; v < a.v_end;
+         v += groups_per_grid) {
         const OffT b = off[v], e = off[v + 1];
-        const bool heavy = (uint64_t)(e - b) >= a.heavy_thr;
+        const bool heavy = a.skip_heavy && (uint64_t)(e - b) >= a.heavy_thr;
         double s = 0;
         if (!heavy)
-            for (OffT k = b + sub; k < e; k += G) s += (double)a.contrib_in[a.src[k]];
+            for (OffT k = b + sub; k < e; k += G) s += (double)a.contrib_in[src[k]];
 #pragma unroll
         for (int d = G / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, G);
         if (sub == 0 && !heavy) pr_finish_vertex(a, v, s, base, res, dang);
@@ -184,9 +254,11 @@ __global__ void k_prref_contrib(const double* val, const uint32_t* outdeg, doubl
 // prSumKernel_static / _dynamic (gpu_kernels.cuh:554-594): sequential FP64
 // sum in CSR order for each active vertex.
 template <class OffT>
-__global__ void k_prref_sum(const OffT* off, const uint32_t* src, const double* contrib, const uint8_t* active,
-                            double* sum, uint64_t V) {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+__global__ void k_prref_sum(const OffT* off, const uint32_t* src, unsigned long long e_base, const double* contrib,
+                            const uint8_t* active, double* sum, uint64_t v_begin, uint64_t v_end) {
+    src -= e_base;
+    for (uint64_t v = v_begin + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < v_end;
+         v += (uint64_t)gridDim.x * blockDim.x) {
         if (!active[v]) continue;
         double s = 0;
         for (OffT k = off[v]; k < off[v + 1]; k++) s += contrib[src[k]];
@@ -240,8 +312,9 @@ static int pr_ensure(asc_graph* g, int iters, bool ref) {
     }
     const int thr = 4096;  // in-degree from which a workgroup sums one vertex
     if (!ref && p->heavy_thr != thr) {
+        // resident vertices only: their in-edges are on the device
         std::vector<uint32_t> hv;
-        for (uint64_t v = 0; v < V; v++)
+        for (uint64_t v = 0; v < g->static_vertices; v++)
             if (g->h.off[v + 1] - g->h.off[v] >= (uint64_t)thr) hv.push_back((uint32_t)v);
         p->n_heavy = (uint32_t)hv.size();
         p->heavy_thr = thr;
@@ -258,7 +331,7 @@ static int pr_ensure(asc_graph* g, int iters, bool ref) {
         ASC_HIP(hipMalloc(&p->count, 4));
         ASC_TRY(p->h_count.alloc(1, true));
     }
-    return ASC_OK;
+    return pr_plan_chunks(g, p);
 }
 
 static int pr_group_for(const asc_graph* g) {
@@ -320,12 +393,28 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         a.damping = (float)o->damping;
         a.damping64 = o->damping;
         a.heavy_thr = (uint32_t)p->heavy_thr;
-        ASC_TRY(launch_pull<OffT>(G, grid, s, a));
+        // resident vertices from the device edges, heavy in-lists by workgroups
+        a.e_base = 0;
+        a.v_begin = 0;
+        a.v_end = g->static_vertices;
+        a.skip_heavy = 1;
+        if (a.v_end > 0) ASC_TRY(launch_pull<OffT>(G, grid, s, a));
         if (p->n_heavy) {
             hipLaunchKernelGGL((k_pr_pull_heavy<OffT>), dim3(std::min<uint32_t>(p->n_heavy, grid)), dim3(kBlock), 0,
                                s, a, p->heavy, p->n_heavy);
             ASC_HIP(hipGetLastError());
         }
+        // on-demand vertices from streamed chunks
+        ASC_TRY(pr_stream_chunks(g, p, [&](const uint32_t* src, uint64_t e0, uint64_t v0, uint64_t v1) {
+            PrArgs c = a;
+            c.src = src;
+            c.e_base = e0;
+            c.v_begin = v0;
+            c.v_end = v1;
+            c.skip_heavy = 0;
+            const int cgrid = (int)std::min<uint64_t>(grid, (v1 - v0 + 63) / 64 + 1);
+            return launch_pull<OffT>(G, cgrid, s, c);
+        }));
     }
     ASC_HIP(hipEventRecord(e1, s));
     ASC_HIP(hipStreamSynchronize(s));
@@ -352,6 +441,9 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         st->reached_degree_sum = g->h.E;
         st->pr_residual = hres[iters - 1];
         st->host_cores = hardware_threads();
+        st->streamed_edges = p->streamed_edges * (uint64_t)iters;
+        st->streamed_bytes = 4 * st->streamed_edges;
+        st->parts = (uint32_t)(p->chunks.size() * (size_t)iters);
         st->kern[3].launches = (uint32_t)iters;
         st->kern[3].ms = kms;
         st->kern[3].edges = g->h.E * (uint64_t)iters;
@@ -382,8 +474,18 @@ static int pr_reference(asc_graph* g, const asc_pr_opts* o, double* rank_out, as
     while (active && it < maxit) {
         it++;
         hipLaunchKernelGGL(k_prref_contrib, dim3(grid), dim3(256), 0, s, p->val64, g->d_outdeg, p->contrib64, V);
-        hipLaunchKernelGGL((k_prref_sum<OffT>), dim3(grid), dim3(256), 0, s, (const OffT*)g->d_off,
-                           (const uint32_t*)g->d_edges, p->contrib64, p->active, p->sum64, V);
+        if (g->static_vertices)
+            hipLaunchKernelGGL((k_prref_sum<OffT>), dim3(grid), dim3(256), 0, s, (const OffT*)g->d_off,
+                               (const uint32_t*)g->d_edges, 0ull, p->contrib64, p->active, p->sum64, 0ull,
+                               (unsigned long long)g->static_vertices);
+        ASC_TRY(pr_stream_chunks(g, p, [&](const uint32_t* src, uint64_t e0, uint64_t v0, uint64_t v1) {
+            const int cgrid = (int)std::min<uint64_t>(grid, (v1 - v0 + 255) / 256);
+            hipLaunchKernelGGL((k_prref_sum<OffT>), dim3(cgrid), dim3(256), 0, s, (const OffT*)g->d_off, src,
+                               (unsigned long long)e0, p->contrib64, p->active, p->sum64, (unsigned long long)v0,
+                               (unsigned long long)v1);
+            ASC_HIP(hipGetLastError());
+            return ASC_OK;
+        }));
         ASC_HIP(hipMemsetAsync(p->count, 0, 4, s));
         hipLaunchKernelGGL(k_prref_update, dim3(grid), dim3(256), 0, s, p->val64, p->sum64, p->active, V, base,
                            damping, o->threshold, p->count);
@@ -411,10 +513,6 @@ int pagerank(asc_graph* g, const asc_pr_opts* o, float* rank_f32, double* rank_f
              asc_stats* st) {
     if (g->h.kind != ASC_KIND_CSC) {
         set_last_error("PageRank needs a BCSC graph (in-CSR + out-degree)");
-        return ASC_ERR_UNSUPPORTED;
-    }
-    if (g->static_vertices < g->h.V) {
-        set_last_error("PageRank over a partially resident graph is not supported");
         return ASC_ERR_UNSUPPORTED;
     }
     if (o->mode == ASC_PR_REFERENCE)

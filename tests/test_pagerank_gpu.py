@@ -60,6 +60,30 @@ def test_pr_rmat_csc(gpu_ctx):
     assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
 
 
+@pytest.mark.parametrize("mode", [_abi.PR_NORMALISED, _abi.PR_REFERENCE])
+def test_pr_partially_resident(mode):
+    """A third of the in-edges resident, the rest streamed in vertex-range
+    chunks every iteration (prSumKernel_dynamic, gpu_kernels.cuh:554-575)."""
+    V = 20000
+    off, dst, _ = G.random_graph(V, 200000, 5, symmetric=False)
+    ioff, isrc, od = csc_of(off, dst, V)
+    E = len(isrc)
+    with A.Context(edge_budget_bytes=4 * (E // 3), part_bytes=4 * 8192) as ctx:
+        with ctx.from_arrays(ioff, isrc, out_degree=od) as g:
+            info = g.info()
+            assert 0 < info["static_vertices"] < V
+            if mode == _abi.PR_NORMALISED:
+                ref, _ = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 20)
+                r, _, st = g.pagerank(iterations=20)
+                assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
+                assert st["streamed_edges"] == info["ondemand_edges"] * 20
+            else:
+                ref, rst = O.pr_ref(O.CSR(ioff, isrc), od)
+                r, _, st = g.pagerank(mode=_abi.PR_REFERENCE)
+                assert st["iterations"] == rst["iterations"]
+                assert np.array_equal(r, ref)
+
+
 @pytest.mark.parametrize("symmetric", [True, False])
 def test_pr_reference_mode_exact(gpu_ctx, symmetric):
     V = 3000
