@@ -278,6 +278,31 @@ class Graph:
     def replan(self, edge_budget_bytes=0, advise_rate=0.0):
         check(_abi.load().asc_graph_replan(self._h, edge_budget_bytes, advise_rate), "asc_graph_replan")
 
+    def autotune(self, budget_ms=30000.0):
+        """Sweeps the tunables on this graph; returns the chosen values."""
+        t = _abi.Tuning()
+        check(_abi.load().asc_autotune(self._h, budget_ms, C.byref(t)), "asc_autotune")
+        self._tuning = t
+        return {n: getattr(t, n) for n, _ in t._fields_ if n != "pad"}
+
+    def save_tuning(self, path):
+        t = getattr(self, "_tuning", None)
+        if t is None:
+            raise RuntimeError("autotune() first")
+        check(_abi.load().asc_tuning_save(path.encode(), self._h, C.byref(t)), "asc_tuning_save")
+
+    def load_tuning(self, path):
+        t = _abi.Tuning()
+        check(_abi.load().asc_tuning_load(path.encode(), self._h, C.byref(t)), "asc_tuning_load")
+        self._tuning = t
+        return {n: getattr(t, n) for n, _ in t._fields_ if n != "pad"}
+
+    def apply_tuning(self, **values):
+        t = _abi.Tuning()
+        for k, v in values.items():
+            setattr(t, k, v)
+        check(_abi.load().asc_tuning_apply(self._h, C.byref(t)), "asc_tuning_apply")
+
     def _run(self, fn, *args, output=True):
         out = np.empty(self.num_vertices, dtype=np.uint32) if output else None
         st = _abi.Stats()

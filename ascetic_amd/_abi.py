@@ -211,10 +211,14 @@ class QStats(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [
         ("heavy_threshold", C.c_int),
+        ("blocks_per_cu", C.c_int),
         ("pr_group", C.c_int),
         ("sssp_delta", C.c_int),
         ("serve_lanes", C.c_int),
+        ("pad", C.c_int),
         ("score_ms", C.c_double),
+        ("untuned_ms", C.c_double),
+        ("fingerprint", C.c_uint64),
     ]
 
 
@@ -262,6 +266,7 @@ SIGNATURES = {
     "asc_autotune": (C.c_int, [P, C.c_double, C.POINTER(Tuning)]),
     "asc_tuning_save": (C.c_int, [C.c_char_p, P, C.POINTER(Tuning)]),
     "asc_tuning_load": (C.c_int, [C.c_char_p, P, C.POINTER(Tuning)]),
+    "asc_tuning_apply": (C.c_int, [P, C.POINTER(Tuning)]),
 }
 
 # diagnostics exported beside the header's functions

@@ -1,8 +1,236 @@
-// Autotuner (placeholder until the sweep lands).
+// Per-graph autotuner: chooses the heavy-vertex (degree-bin) threshold, the
+// launch width, the SSSP band width, the PageRank lanes per vertex and the
+// serving batch width by timing a sampled workload on the graph itself.
+//
+// Coordinate descent, one parameter at a time in that order, each candidate
+// timed as the minimum of two runs of the workload: BFS (and SSSP for
+// weighted graphs) from the maximum-degree vertex and three seeded random
+// vertices with out-edges, or three normalised PageRank iterations for BFS
+// CSC graphs.  The serving width is timed as the drain time of a burst of
+// 2 x width BFS queries.  The sweep stops early when `budget_ms` is spent,
+// keeping the best values found.  Results live on the graph (they are read at
+// launch time, nothing is re-uploaded) and can be saved in a tuning table
+// keyed by a fingerprint of the graph.
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <limits>
+#include <sstream>
+
 #include "internal.hpp"
 
-extern "C" {
-int asc_autotune(asc_graph*, double, asc_tuning*) { return ASC_ERR_UNSUPPORTED; }
-int asc_tuning_save(const char*, const asc_graph*, const asc_tuning*) { return ASC_ERR_UNSUPPORTED; }
-int asc_tuning_load(const char*, asc_graph*, asc_tuning*) { return ASC_ERR_UNSUPPORTED; }
+namespace asc {
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
 }
+
+// Hash of the size, kind and 4096 sampled offsets and edge words.
+uint64_t fingerprint(const asc_graph* g) {
+    const HostCSR& h = g->h;
+    uint64_t f = mix64(h.V * 0x9E3779B97F4A7C15ull ^ h.E ^ ((uint64_t)h.kind << 60));
+    const uint64_t words = h.E * h.edge_words();
+    for (uint64_t i = 0; i < 4096; i++) {
+        f = mix64(f ^ h.off[(i * (h.V + 1)) / 4096]);
+        if (words) f = mix64(f ^ h.edges[(i * words) / 4096]);
+    }
+    return f;
+}
+
+struct Workload {
+    asc_graph* g;
+    std::vector<uint32_t> srcs;
+
+    double run_once() {
+        const double t0 = now_ms();
+        int r = ASC_OK;
+        if (g->h.kind == ASC_KIND_CSC) {
+            asc_pr_opts o;
+            asc_pr_opts_init(&o);
+            o.iterations = 3;
+            r = asc_pr(g, &o, nullptr, nullptr, nullptr, nullptr);
+        } else {
+            for (uint32_t s : srcs) {
+                if (r == ASC_OK) r = asc_bfs(g, s, nullptr, nullptr);
+                if (r == ASC_OK && g->h.kind == ASC_KIND_WCSR) r = asc_sssp(g, s, nullptr, nullptr);
+            }
+        }
+        return r == ASC_OK ? now_ms() - t0 : std::numeric_limits<double>::infinity();
+    }
+    double time() { return std::min(run_once(), run_once()); }
+};
+
+double time_serving(asc_graph* g, const std::vector<uint32_t>& srcs, int lanes) {
+    asc_server_opts o;
+    asc_server_opts_init(&o);
+    o.lanes = lanes;
+    asc_server* s = nullptr;
+    if (asc_server_create(g, &o, &s) != ASC_OK) return std::numeric_limits<double>::infinity();
+    const double t0 = now_ms();
+    std::vector<uint64_t> tickets;
+    int r = ASC_OK;
+    for (int i = 0; i < 2 * lanes && r == ASC_OK; i++) {
+        uint64_t t;
+        r = asc_submit(s, ASC_QUERY_BFS, srcs[(size_t)i % srcs.size()], nullptr, &t);
+        if (r == ASC_OK) tickets.push_back(t);
+    }
+    for (uint64_t t : tickets) asc_wait(s, t, nullptr);
+    const double ms = now_ms() - t0;
+    asc_server_destroy(s);
+    // per query, so that widths are compared at equal work
+    return r == ASC_OK ? ms / (2.0 * lanes) : std::numeric_limits<double>::infinity();
+}
+
+}  // namespace
+}  // namespace asc
+
+using namespace asc;
+
+extern "C" {
+
+int asc_tuning_apply(asc_graph* g, const asc_tuning* t) {
+    if (!g || !t) return ASC_ERR_INVALID;
+    if (t->heavy_threshold > 0) g->heavy_threshold = t->heavy_threshold;
+    if (t->blocks_per_cu > 0) g->blocks_per_cu = t->blocks_per_cu;
+    if (t->pr_group > 0) g->pr_group = t->pr_group;
+    if (t->sssp_delta != 0) g->sssp_delta = t->sssp_delta;
+    if (t->serve_lanes > 0) g->serve_lanes = std::min(t->serve_lanes, 64);
+    return ASC_OK;
+}
+
+int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out) {
+    if (!g) return ASC_ERR_INVALID;
+    if (budget_ms <= 0) budget_ms = 30000;
+    const double t_start = now_ms();
+    auto over = [&] { return now_ms() - t_start > budget_ms; };
+    const uint64_t fp = fingerprint(g);
+
+    Workload w{g, {}};
+    if (g->h.kind != ASC_KIND_CSC) {
+        w.srcs.push_back(g->max_degree_vertex);
+        uint64_t x = fp;
+        for (int tries = 0; w.srcs.size() < 4 && tries < 1000; tries++) {
+            x = mix64(x + tries);
+            const uint32_t v = (uint32_t)(x % g->h.V);
+            if (g->h.off[v + 1] > g->h.off[v]) w.srcs.push_back(v);
+        }
+    }
+    (void)w.run_once();  // allocate state, warm caches
+    const double untuned = w.time();
+    if (!std::isfinite(untuned)) return ASC_ERR_DEVICE;
+    double best = untuned;
+
+    auto sweep = [&](int* field, std::initializer_list<int> cands) {
+        int keep = *field;
+        for (int c : cands) {
+            if (over()) break;
+            if (c == keep) continue;
+            *field = c;
+            const double ms = w.time();
+            if (ms < best * 0.98) {  // require a clear win over noise
+                best = ms;
+                keep = c;
+            }
+        }
+        *field = keep;
+    };
+    if (g->h.kind != ASC_KIND_CSC) {
+        sweep(&g->heavy_threshold, {256, 512, 1024, 4096, 16384, 65536});
+        sweep(&g->blocks_per_cu, {2, 4, 16});
+        if (g->h.kind == ASC_KIND_WCSR) {
+            const int mw = std::max(1, (int)std::lround(g->mean_weight > 0 ? g->mean_weight : 32.0));
+            sweep(&g->sssp_delta, {-1, std::max(1, mw / 2), mw, 4 * mw});
+        }
+    } else {
+        sweep(&g->pr_group, {4, 8, 16, 32, 64});
+    }
+    if (g->h.kind != ASC_KIND_CSC && g->static_vertices == g->h.V && !over()) {
+        double best_q = std::numeric_limits<double>::infinity();
+        int best_l = g->serve_lanes;
+        for (int lanes : {16, 32, 64}) {
+            if (over()) break;
+            const double q = time_serving(g, w.srcs, lanes);
+            if (q < best_q) {
+                best_q = q;
+                best_l = lanes;
+            }
+        }
+        g->serve_lanes = best_l;
+    }
+    if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->heavy_threshold = g->heavy_threshold > 0 ? g->heavy_threshold : effective_heavy_threshold(g);
+        out->blocks_per_cu = g->blocks_per_cu > 0 ? g->blocks_per_cu : 8;
+        out->pr_group = g->pr_group;
+        out->sssp_delta = g->sssp_delta;
+        out->serve_lanes = g->serve_lanes;
+        out->score_ms = best;
+        out->untuned_ms = untuned;
+        out->fingerprint = fp;
+    }
+    return ASC_OK;
+}
+
+int asc_tuning_save(const char* path, const asc_graph* g, const asc_tuning* t) {
+    if (!path || !g || !t) return ASC_ERR_INVALID;
+    const uint64_t fp = fingerprint(g);
+    std::vector<std::string> keep;
+    {
+        std::ifstream in(path);
+        std::string line;
+        char key[64];
+        snprintf(key, sizeof(key), "fingerprint %016llx", (unsigned long long)fp);
+        while (std::getline(in, line))
+            if (!line.empty() && line.rfind(key, 0) != 0) keep.push_back(line);
+    }
+    std::ofstream outf(path, std::ios::trunc);
+    if (!outf) return ASC_ERR_IO;
+    for (auto& l : keep) outf << l << "\n";
+    char buf[512];
+    snprintf(buf, sizeof(buf),
+             "fingerprint %016llx V %llu E %llu heavy_threshold %d blocks_per_cu %d pr_group %d sssp_delta %d "
+             "serve_lanes %d score_ms %.3f untuned_ms %.3f",
+             (unsigned long long)fp, (unsigned long long)g->h.V, (unsigned long long)g->h.E, t->heavy_threshold,
+             t->blocks_per_cu, t->pr_group, t->sssp_delta, t->serve_lanes, t->score_ms, t->untuned_ms);
+    outf << buf << "\n";
+    return outf.good() ? ASC_OK : ASC_ERR_IO;
+}
+
+int asc_tuning_load(const char* path, asc_graph* g, asc_tuning* t) {
+    if (!path || !g) return ASC_ERR_INVALID;
+    std::ifstream in(path);
+    if (!in) return ASC_ERR_IO;
+    const uint64_t fp = fingerprint(g);
+    std::string line;
+    while (std::getline(in, line)) {
+        std::istringstream ss(line);
+        std::string k;
+        asc_tuning x;
+        std::memset(&x, 0, sizeof(x));
+        bool match = false;
+        while (ss >> k) {
+            std::string v;
+            if (!(ss >> v)) break;
+            if (k == "fingerprint") match = std::stoull(v, nullptr, 16) == fp, x.fingerprint = fp;
+            else if (k == "heavy_threshold") x.heavy_threshold = std::stoi(v);
+            else if (k == "blocks_per_cu") x.blocks_per_cu = std::stoi(v);
+            else if (k == "pr_group") x.pr_group = std::stoi(v);
+            else if (k == "sssp_delta") x.sssp_delta = std::stoi(v);
+            else if (k == "serve_lanes") x.serve_lanes = std::stoi(v);
+            else if (k == "score_ms") x.score_ms = std::stod(v);
+            else if (k == "untuned_ms") x.untuned_ms = std::stod(v);
+        }
+        if (match) {
+            asc_tuning_apply(g, &x);
+            if (t) *t = x;
+            return ASC_OK;
+        }
+    }
+    set_last_error("no tuning line for this graph in %s", path);
+    return ASC_ERR_STATE;
+}
+
+}  // extern "C"

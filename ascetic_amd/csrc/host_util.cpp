@@ -161,7 +161,7 @@ void asc_gen_params_init(asc_gen_params* p) {
 
 void asc_server_opts_init(asc_server_opts* o) {
     if (!o) return;
-    o->lanes = 64;
+    o->lanes = 0;  // the graph's tuned batch width (64 until tuned)
     o->state_budget_bytes = 0;
     o->queue_capacity = 4096;
     o->pad = 0;

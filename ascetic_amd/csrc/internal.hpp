@@ -194,6 +194,7 @@ struct asc_graph {
     int sssp_delta = 0;
     double mean_weight = 0;         // SSSP: mean edge weight (0 = not computed)
     int serve_lanes = 64;
+    int blocks_per_cu = 0;          // workgroups per CU of traversal launches (0 = 8)
     uint32_t max_degree = 0, max_degree_vertex = 0;
     asc::TravState* trav = nullptr;
     asc::PrState* pr = nullptr;

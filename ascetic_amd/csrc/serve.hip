@@ -546,7 +546,7 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
 // Resets and seeds newly admitted lanes, then launches one step.
 template <class OffT, class EdgeT>
 static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
-    const int grid = g->ctx->num_cus * kBlocksPerCU;
+    const int grid = g->ctx->num_cus * (g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
     const uint32_t unreached = (uint32_t)(e.V + 1);
     auto a = engine_args<OffT, EdgeT>(g, e);
     if (!admitted.empty()) {
@@ -702,7 +702,8 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
     }
     asc_server_opts o;
     if (opts) o = *opts; else asc_server_opts_init(&o);
-    if (o.lanes <= 0 || o.lanes > kMaxLanes) o.lanes = kMaxLanes;
+    if (o.lanes <= 0) o.lanes = g->serve_lanes > 0 ? g->serve_lanes : kMaxLanes;  // tuned batch width
+    if (o.lanes > kMaxLanes) o.lanes = kMaxLanes;
     if (o.queue_capacity <= 0) o.queue_capacity = 4096;
     ASC_HIP(hipSetDevice(g->ctx->device));
     const uint64_t V = g->h.V;

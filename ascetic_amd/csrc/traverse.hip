@@ -694,7 +694,7 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     hipStream_t s = ctx->s_main;
     ASC_HIP(hipSetDevice(ctx->device));
     const int fill_grid = ctx->num_cus * 4;
-    const int grid = ctx->num_cus * kBlocksPerCU;
+    const int grid = ctx->num_cus * (g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
     const bool prof = ctx->cfg.profile != 0;
     const uint32_t delta = ALGO == ALG_SSSP ? sssp_delta(g) : 0;
 

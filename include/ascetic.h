@@ -318,7 +318,8 @@ int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph**
 typedef enum { ASC_QUERY_BFS = 0, ASC_QUERY_SSSP = 1 } asc_query_kind;
 
 typedef struct {
-    int lanes;                   /* max queries in flight per kind (1..64), 0 = 64 */
+    int lanes;                   /* max queries in flight per kind (1..64);
+                                    0 = the graph's tuned width (64 untuned) */
     uint64_t state_budget_bytes; /* device bytes for per-query state (4 V bytes
                                     per lane); 0 = half the free memory       */
     int queue_capacity;          /* max submitted-but-unfinished queries      */
@@ -358,17 +359,29 @@ int asc_server_get_stats(asc_server* s, asc_server_stats* st);
 
 /* ---- autotuning -------------------------------------------------------- */
 typedef struct {
-    int heavy_threshold;
-    int pr_group;
-    int sssp_delta;
-    int serve_lanes;
-    double score_ms;             /* best measured time of the tuned workload  */
+    int heavy_threshold;         /* degree from which a frontier vertex is
+                                    expanded grid-cooperatively              */
+    int blocks_per_cu;           /* workgroups per CU of traversal launches   */
+    int pr_group;                /* lanes per vertex of the PageRank pull     */
+    int sssp_delta;              /* SSSP band width (-1 = no far pile)        */
+    int serve_lanes;             /* serving batch width                       */
+    int pad;
+    double score_ms;             /* summed time of the sampled workload with
+                                    the chosen values                         */
+    double untuned_ms;           /* the same workload with the defaults       */
+    uint64_t fingerprint;        /* graph the values were tuned on            */
 } asc_tuning;
-/* Sweeps the tunables on sampled sources of g and stores the best values in
- * the graph (and in *out).  `budget_ms` bounds the sweep's wall time. */
+/* Sweeps the tunables on sampled sources of g (coordinate descent, one
+ * parameter at a time), stores the best values in the graph and in *out.
+ * `budget_ms` bounds the sweep's wall time (0 = 30 s). */
 int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out);
+/* Tuning tables: one line per graph fingerprint, "key value" pairs.  Save
+ * replaces the line of this graph; load applies the line matching the
+ * graph's fingerprint or returns ASC_ERR_STATE when there is none. */
 int asc_tuning_save(const char* path, const asc_graph* g, const asc_tuning* t);
 int asc_tuning_load(const char* path, asc_graph* g, asc_tuning* t);
+/* Applies explicit values to a graph (0 keeps a field unchanged). */
+int asc_tuning_apply(asc_graph* g, const asc_tuning* t);
 
 #ifdef __cplusplus
 }
