@@ -130,6 +130,70 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0):
     return out
 
 
+def _kernel_rate(st, name, s_e, s_o):
+    k = st["kernels"][name]
+    if not k["ms"]:
+        return None
+    if name == "pr_pull":
+        b = k["edges"] * 8 + k["vertices"] * (2 * s_o + 16)
+    else:
+        b = algorithmic_bytes(st["kernels"], s_e, s_o)[name]
+    return round(b / (k["ms"] / 1e3) / 1e9, 1)
+
+
+def per_algo_bench(A, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
+    """GTEPS per algorithm: BFS on R-MAT (C1 shape) and CC / PageRank on an
+    Erdos-Renyi graph (C3 shape), fully resident."""
+    out = {}
+    # BFS: R-MAT scale 20, ef 16, symmetrized, u32 offsets, source 0
+    t = time.perf_counter()
+    h = A.rmat(bfs_scale, 16, seed=1)
+    off, dst = h.offsets().copy(), h.dst().copy()
+    log(f"bfs graph: R-MAT s{bfs_scale} V={h.num_vertices} E={h.num_edges} ({time.perf_counter() - t:.1f}s)")
+    with A.Context(profile=True) as ctx:
+        g = ctx.adopt(h, offset_bytes=4)
+        lv, st = g.bfs(0)
+        runs = [g.bfs(0, output=False)[1] for _ in range(steps)]
+        ms = float(np.median([r["total_ms"] for r in runs]))
+        gteps = st["reached_degree_sum"] / 2 / (ms / 1e3) / 1e9
+        csr = O.CSR(off, dst)
+        tc = time.perf_counter()
+        ref, _ = O.bfs_ref(csr, 0, nthreads=cpu_threads)
+        tcpu = time.perf_counter() - tc
+        out["bfs"] = {"workload": f"bfs_rmat_s{bfs_scale}_ef16_src0_u32off", "gteps": round(gteps, 3),
+                      "ms": round(ms, 3), "levels": st["iterations"], "reached": st["reached"],
+                      "exact": bool(np.array_equal(lv, ref)),
+                      "expand_local_GBps": _kernel_rate(runs[-1], "expand_local", 4, 4),
+                      "cpu_port_gteps": round(st["reached_degree_sum"] / 2 / tcpu / 1e9, 3), "cpu_cores": cpu_threads}
+        g.close()
+    del h, off, dst
+    # CC and PageRank: Erdos-Renyi, 2^er_log_v vertices, 2^er_log_e undirected
+    # pairs, symmetrized (so the out-CSR is also the in-CSR PageRank pulls over)
+    t = time.perf_counter()
+    h = A.erdos_renyi(1 << er_log_v, 1 << er_log_e, seed=3)
+    V, E = h.num_vertices, h.num_edges
+    off, dst = h.offsets(), h.dst()
+    log(f"er graph: V={V} E={E} ({time.perf_counter() - t:.1f}s)")
+    with A.Context(profile=True) as ctx:
+        with ctx.from_arrays(off, dst) as g:
+            _, st = g.cc(output=False)
+            ccs = [g.cc(output=False)[1] for _ in range(max(1, steps // 2))]
+            ms = float(np.median([c["total_ms"] for c in ccs]))
+            out["cc"] = {"workload": f"cc_er_v2^{er_log_v}_e2^{er_log_e}", "gteps": round(E / 2 / (ms / 1e3) / 1e9, 3),
+                         "ms": round(ms, 3), "iterations": st["iterations"],
+                         "expand_local_GBps": _kernel_rate(ccs[-1], "expand_local", 4, 8)}
+        outdeg = np.diff(off).astype(np.uint32)
+        with ctx.from_arrays(off, dst, out_degree=outdeg) as g:
+            _, res, _ = g.pagerank(iterations=20)
+            prs = [g.pagerank(iterations=20)[2] for _ in range(max(1, steps // 2))]
+            ms = float(np.median([p["total_ms"] for p in prs]))
+            out["pagerank"] = {"workload": f"pr_er_v2^{er_log_v}_e2^{er_log_e}_20it_fp32",
+                               "gteps": round(E * 20 / (ms / 1e3) / 1e9, 3), "ms": round(ms, 3),
+                               "final_residual": float(res[-1]), "pr_pull_GBps": _kernel_rate(prs[-1], "pr_pull", 4, 8)}
+    h.free()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +209,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving section (0 = skip)")
+    ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
+    ap.add_argument("--algo-er", type=str, default="26,30",
+                    help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
     ap.add_argument("--serve-queries", type=int, default=512)
     args = ap.parse_args()
 
@@ -274,6 +341,12 @@ def main():
     g.close()
     ctx.close()
     del h, off, ew
+    if args.algo_bfs_scale > 0 and args.algo_er:
+        from oracle import oracle as O
+        lv, le = (int(x) for x in args.algo_er.split(","))
+        result["per_algo"] = per_algo_bench(A, O, args.algo_bfs_scale, lv, le, args.steps,
+                                            args.cpu_threads or min(16, os.cpu_count() or 1))
+        result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
     if args.serve_scale > 0:
         result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0])
     print(json.dumps(result), flush=True)
