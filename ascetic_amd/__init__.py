@@ -213,6 +213,12 @@ class Context:
     def synchronize(self):
         check(_abi.load().asc_ctx_synchronize(self._h), "asc_ctx_synchronize")
 
+    def copy_probe(self, nbytes):
+        """One plain device copy of nbytes (counter calibration); returns ms."""
+        ms = C.c_double()
+        check(_abi.load().asc_copy_probe(self._h, nbytes, C.byref(ms)), "asc_copy_probe")
+        return ms.value
+
     def measure_link(self, nbytes=256 << 20, reps=8):
         """Pinned host <-> device copy bandwidth in GB/s: (h2d, d2h)."""
         h2d, d2h = C.c_double(), C.c_double()

@@ -235,6 +235,7 @@ SIGNATURES = {
     "asc_device_count": (C.c_int, []),
     "asc_ctx_synchronize": (C.c_int, [P]),
     "asc_measure_link": (C.c_int, [P, C.c_uint64, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "asc_copy_probe": (C.c_int, [P, C.c_uint64, C.POINTER(C.c_double)]),
     "asc_graph_load": (C.c_int, [P, C.c_char_p, C.c_int, PP]),
     "asc_graph_from_csr": (C.c_int, [P, C.POINTER(CsrView), PP]),
     "asc_graph_destroy": (C.c_int, [P]),

@@ -166,9 +166,46 @@ static int finish_graph(asc_ctx* ctx, asc_graph* g, asc_graph** out) {
 
 }  // namespace asc
 
+namespace asc {
+__global__ void k_copy_probe(const uint4* __restrict__ src, uint4* __restrict__ dst, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace asc
+
 using namespace asc;
 
 extern "C" {
+
+int asc_copy_probe(asc_ctx* ctx, uint64_t bytes, double* ms) {
+    if (!ctx || bytes < 16) return ASC_ERR_INVALID;
+    ASC_HIP(hipSetDevice(ctx->device));
+    const uint64_t n = bytes / 16;
+    void *src = nullptr, *dst = nullptr;
+    if (hipMalloc(&src, 16 * n) != hipSuccess) return ASC_ERR_NOMEM;
+    if (hipMalloc(&dst, 16 * n) != hipSuccess) {
+        (void)hipFree(src);
+        return ASC_ERR_NOMEM;
+    }
+    hipEvent_t a, b;
+    ASC_HIP(hipEventCreate(&a));
+    ASC_HIP(hipEventCreate(&b));
+    ASC_HIP(hipMemsetAsync(src, 1, 16 * n, ctx->s_main));
+    ASC_HIP(hipEventRecord(a, ctx->s_main));
+    hipLaunchKernelGGL(k_copy_probe, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->s_main, (const uint4*)src, (uint4*)dst,
+                       n);
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipEventRecord(b, ctx->s_main));
+    ASC_HIP(hipEventSynchronize(b));
+    float t = 0;
+    (void)hipEventElapsedTime(&t, a, b);
+    if (ms) *ms = t;
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(src);
+    (void)hipFree(dst);
+    return ASC_OK;
+}
 
 int asc_device_count(void) {
     int n = 0;

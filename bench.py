@@ -213,6 +213,8 @@ def main():
     ap.add_argument("--algo-er", type=str, default="26,30",
                     help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
     ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also launch one plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -266,6 +268,18 @@ def main():
     achieved = abytes[dom] / (dk["ms"] / 1e3) / 1e9 if dk["ms"] > 0 else 0.0
     streamed = sum(r["streamed_bytes"] for r in runs) / args.steps
     link_h2d, link_d2h = ctx.measure_link(256 << 20, 8)
+    if args.copy_probe_mb > 0:
+        ctx.copy_probe(args.copy_probe_mb << 20)
+    workload = f"sssp_rmat_s{args.scale}_ef{args.edge_factor}_w1-64_budget{int(args.budget_frac * 100)}"
+    # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"{workload}_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            tj = json.load(f)
+        if tj.get("kernel") == dom:
+            traffic = tj.get("hbm_bytes_per_launch")
 
     result = {
         "metric": METRIC,
@@ -281,7 +295,7 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {
-            "workload": f"sssp_rmat_s{args.scale}_ef{args.edge_factor}_w1-64_budget{int(args.budget_frac * 100)}",
+            "workload": workload,
             "graph": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, (0.57,0.19,0.19,0.05), "
                      f"symmetrized, seed {args.seed}",
             "V": V, "E_directed": E, "source": src, "offsets": "u64", "edge_record": "{u32 dst, u32 w}",
@@ -297,7 +311,7 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "launches_per_step": dk["launches"] / args.steps,
             "algorithmic_bytes_per_launch": abytes[dom] / max(dk["launches"], 1),
             "avg_launch_us": dk["ms"] * 1e3 / max(dk["launches"], 1),

@@ -99,7 +99,7 @@ typedef struct {
                                     bytes of edge records (overrides
                                     advise_rate); 0: derived from free memory */
     uint64_t part_bytes;         /* bytes of each of the two on-demand part
-                                    buffers; 0 = 256 MiB                      */
+                                    buffers; 0 = 64 MiB                       */
     uint64_t device_mem_limit;   /* > 0: pretend the device has this much free
                                     memory when sizing (testing); 0 = query   */
     int gather_threads;          /* host gather threads; 0 = min(16, cores)   */
@@ -133,6 +133,10 @@ int asc_ctx_synchronize(asc_ctx* ctx);
  * device `reps` times in each direction (HIP events on the copy stream) and
  * reports GB/s; the roofline for streamed on-demand edges. */
 int asc_measure_link(asc_ctx* ctx, uint64_t bytes, int reps, double* h2d_gbps, double* d2h_gbps);
+/* Calibration kernel for hardware counters (no counterpart in the
+ * reference): one launch of a plain device copy (kernel `k_copy_probe`) that
+ * reads and writes exactly `bytes` each; *ms is its duration. */
+int asc_copy_probe(asc_ctx* ctx, uint64_t bytes, double* ms);
 
 /* ---- graphs ------------------------------------------------------------ */
 /* Host view of a CSR/CSC graph.  Exactly one of off64/off32 is set; it holds
