@@ -183,7 +183,8 @@ class Context:
 
     def __init__(self, device=0, advise_rate=0.0, edge_budget_bytes=0, part_bytes=0, device_mem_limit=0,
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
-                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=True):
+                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=True,
+                 bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -194,6 +195,7 @@ class Context:
             heavy_threshold, pr_group, sssp_delta, int(verbose))
         cfg.profile = int(profile)
         cfg.stream_pack = int(stream_pack)
+        cfg.bfs_direction, cfg.cc_method = bfs_direction, cc_method
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")
         self._h = h

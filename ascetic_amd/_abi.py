@@ -28,6 +28,8 @@ KIND_CSR, KIND_WCSR, KIND_CSC = range(3)
 VALUES_REFERENCE, VALUES_ZERO_BASED = 0, 1
 STREAM_GATHER, STREAM_ZEROCOPY = 0, 1
 PR_NORMALISED, PR_REFERENCE = 0, 1
+BFS_AUTO, BFS_TOP_DOWN = 0, 1
+CC_AUTO, CC_LABEL_PROP = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1
 
 u64p = C.POINTER(C.c_uint64)
@@ -50,6 +52,8 @@ class Config(C.Structure):
         ("verbose", C.c_int),
         ("profile", C.c_int),
         ("stream_pack", C.c_int),
+        ("bfs_direction", C.c_int),
+        ("cc_method", C.c_int),
     ]
 
 
@@ -79,6 +83,8 @@ class GraphInfo(C.Structure):
         ("device_bytes", C.c_uint64),
         ("max_degree", C.c_uint32),
         ("max_degree_vertex", C.c_uint32),
+        ("symmetric", C.c_int),
+        ("pad", C.c_int),
     ]
 
 
@@ -92,7 +98,8 @@ class KernelStats(C.Structure):
     ]
 
 
-KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull")
+KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull", "bottom_up", "union_find")
+PATH_NAMES = ("frontier", "direction_opt", "union_find")
 
 
 class Stats(C.Structure):
@@ -116,13 +123,15 @@ class Stats(C.Structure):
         ("kernel_ms", C.c_double),
         ("pr_residual", C.c_double),
         ("host_cores", C.c_int),
-        ("kern", KernelStats * 4),
+        ("path", C.c_int),
+        ("kern", KernelStats * 6),
     ]
 
     def as_dict(self):
         d = {name: getattr(self, name) for name, _ in self._fields_ if name != "kern"}
+        d["path"] = PATH_NAMES[self.path] if 0 <= self.path < len(PATH_NAMES) else self.path
         d["kernels"] = {KERNEL_NAMES[i]: {n: getattr(self.kern[i], n) for n in ("launches", "ms", "edges", "vertices")}
-                        for i in range(4)}
+                        for i in range(len(KERNEL_NAMES))}
         return d
 
 

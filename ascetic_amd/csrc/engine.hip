@@ -131,6 +131,10 @@ int graph_upload(asc_graph* g) {
             std::memcpy(g->zc_tail.p + b, src + b, 4 * (en - b));
         });
     }
+    // symmetry decides the resident fast paths (direction-optimizing BFS,
+    // union-find CC); checked on the device copy when every edge is on it
+    g->symmetric = -1;
+    if (g->static_vertices == V && g->h.kind != ASC_KIND_CSC) ASC_TRY(check_symmetric(g));
     return ASC_OK;
 }
 
@@ -369,6 +373,7 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info) {
     info->device_bytes = g->device_bytes;
     info->max_degree = g->max_degree;
     info->max_degree_vertex = g->max_degree_vertex;
+    info->symmetric = g->symmetric;
     return ASC_OK;
 }
 

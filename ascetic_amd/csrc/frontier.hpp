@@ -1,0 +1,200 @@
+// Frontier relax kernels shared by the traversal engine (traverse.hip) and the
+// resident fast paths (resident.hip): check-then-atomicMin relaxation with
+// routing of improved vertices, and the edge-balanced expansion of a resident
+// frontier queue (k_expand_local) and of its heavy vertices (k_expand_heavy).
+#pragma once
+
+#include "device_common.hpp"
+#include "traverse.hpp"
+
+namespace asc {
+
+enum { TO_NONE = 0, TO_RES = 1, TO_PEND = 2, TO_FAR = 3 };
+
+template <int ALGO>
+__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
+    return ALGO == ALG_SSSP ? sv + w : sv;  // BFS passes level + 2 as sv
+}
+
+// Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
+// and routing of an improved vertex.
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
+    if (ALGO == ALG_BFS) {
+        const uint32_t bit = 1u << (v & 31);
+        uint32_t* word = a.visited + (v >> 5);
+        if (*word & bit) return TO_NONE;
+        if (atomicOr(word, bit) & bit) return TO_NONE;
+        a.val[v] = cand;
+        return v < a.vs ? TO_RES : TO_PEND;
+    } else {
+        if (cand >= a.val[v]) return TO_NONE;
+        const uint32_t old = atomicMin(a.val + v, cand);
+        if (old <= cand) return TO_NONE;
+        if (ALGO == ALG_SSSP && cand >= a.theta)
+            return atomicMax(a.fstamp + v, a.fepoch) < a.fepoch ? TO_FAR : TO_NONE;
+        if (v < a.vs) return atomicMax(a.stamp + v, level + 1) < level + 1 ? TO_RES : TO_NONE;
+        return atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
+    }
+}
+
+template <int ALGO>
+struct Pushes {
+    WaveStage<kStageCap> res, pend, far;
+
+    template <class A>
+    __device__ __forceinline__ void push(const A& a, Slot* nxt, int to, uint32_t v) {
+        res.push(to == TO_RES, v, a.q_out, &nxt->n_res);
+        pend.push(to == TO_PEND, v, a.pend_out, a.pend_count);
+        if (ALGO == ALG_SSSP) far.push(to == TO_FAR, v, a.far_out, a.far_count);
+    }
+    template <class A>
+    __device__ __forceinline__ void flush(const A& a, Slot* nxt) {
+        res.flush(a.q_out, &nxt->n_res);
+        pend.flush(a.pend_out, a.pend_count);
+        if (ALGO == ALG_SSSP) far.flush(a.far_out, a.far_count);
+    }
+};
+
+#define ASC_STAGE_DECL                                              \
+    __shared__ uint32_t s_stage[3][kWavesPerBlock][kStageCap];      \
+    Pushes<ALGO> pq{{s_stage[0][wave_id()], 0}, {s_stage[1][wave_id()], 0}, {s_stage[2][wave_id()], 0}}
+
+template <int ALGO, class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    __shared__ uint32_t s_pref[kBlock];
+    __shared__ OffT s_beg[kBlock];
+    __shared__ uint32_t s_sv[kBlock];
+    __shared__ uint32_t s_tmp[kWavesPerBlock];
+    ASC_STAGE_DECL;
+
+    Slot* cur = a.slots + (level & 3);
+    Slot* nxt = a.slots + ((level + 1) & 3);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        Slot* z = a.slots + ((level + 2) & 3);
+        z->n_res = 0;
+        z->heavy = 0;
+    }
+    const uint32_t n = cur->n_res;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    unsigned long long my_edges = 0, my_verts = 0;
+
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        const uint32_t i = base + tid;
+        uint32_t deg = 0, sv = 0;
+        OffT beg = 0;
+        if (i < n) {
+            const uint32_t u = a.q_in[i];
+            beg = a.off[u];
+            uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+            if (ALGO == ALG_BFS) {
+                sv = level + 2;
+            } else {
+                sv = a.val[u];
+                if (a.done[u] == sv) d = 0;  // already expanded at this value
+                else a.done[u] = sv;
+            }
+            if (d >= a.heavy_thr) {
+                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
+                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
+                a.heavy_u[idx] = u;
+                a.heavy_sv[idx] = sv;
+                a.heavy_base[idx] = p & kHeavyMask;
+            } else {
+                deg = (uint32_t)d;
+            }
+        }
+        uint32_t total;
+        const uint32_t pre = block_scan_excl(deg, s_tmp, total);
+        s_pref[tid] = pre;
+        s_beg[tid] = beg;
+        s_sv[tid] = sv;
+        __syncthreads();
+        const uint32_t cnt = min(n - base, (uint32_t)kBlock);
+        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
+            const uint32_t e = eb + lane;
+            int to = TO_NONE;
+            uint32_t v = 0;
+            if (e < total) {
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t step = kBlock / 2; step > 0; step >>= 1)
+                    if (k + step < cnt && s_pref[k + step] <= e) k += step;
+                const EdgeT ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
+                v = edge_dst(ed);
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+            }
+            pq.push(a, nxt, to, v);
+        }
+        my_edges += total;
+        my_verts += cnt;
+        __syncthreads();
+    }
+    pq.flush(a, nxt);
+    if (tid == 0 && my_verts) {
+        atomicAdd(&a.stats->edges_local, my_edges);
+        atomicAdd(&a.stats->verts_local, my_verts);
+    }
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    constexpr int kWin = 8;  // a tile spans <= kTile / kMinHeavy + 1 heavy entries
+    __shared__ unsigned long long s_hb[kWin];
+    __shared__ OffT s_hoff[kWin];
+    __shared__ uint32_t s_hsv[kWin];
+    __shared__ uint32_t s_k0;
+    ASC_STAGE_DECL;
+
+    Slot* cur = a.slots + (level & 3);
+    Slot* nxt = a.slots + ((level + 1) & 3);
+    const unsigned long long h = cur->heavy;
+    const uint32_t nh = (uint32_t)(h >> kHeavyShift);
+    const uint64_t total = h & kHeavyMask;
+    if (nh == 0) return;
+    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
+    unsigned long long my_edges = 0;
+
+    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
+        if (tid == 0) {
+            uint32_t lo = 0, hi = nh - 1;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi + 1) >> 1;
+                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+            }
+            s_k0 = lo;
+        }
+        __syncthreads();
+        const uint32_t k0 = s_k0;
+        const uint32_t nwin = min(nh - k0, (uint32_t)kWin);
+        if (tid < nwin) {
+            s_hb[tid] = a.heavy_base[k0 + tid];
+            s_hoff[tid] = a.off[a.heavy_u[k0 + tid]];
+            s_hsv[tid] = a.heavy_sv[k0 + tid];
+        }
+        __syncthreads();
+        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
+        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
+            const uint32_t rr = r + lane;
+            int to = TO_NONE;
+            uint32_t v = 0;
+            if (rr < tlen) {
+                const uint64_t e = t0 + rr;
+                uint32_t k = 0;
+#pragma unroll
+                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
+                    if (j < nwin && s_hb[j] <= e) k = j;
+                const EdgeT ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
+                v = edge_dst(ed);
+                to = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
+            }
+            pq.push(a, nxt, to, v);
+        }
+        my_edges += tlen;
+        __syncthreads();
+    }
+    pq.flush(a, nxt);
+    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
+}
+
+}  // namespace asc

@@ -129,6 +129,8 @@ void asc_config_init(asc_config* c) {
     c->verbose = 0;
     c->profile = 0;
     c->stream_pack = 1;
+    c->bfs_direction = ASC_BFS_AUTO;
+    c->cc_method = ASC_CC_AUTO;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

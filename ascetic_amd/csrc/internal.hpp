@@ -196,6 +196,9 @@ struct asc_graph {
     int serve_lanes = 64;
     int blocks_per_cu = 0;          // workgroups per CU of traversal launches (0 = 8)
     uint32_t max_degree = 0, max_degree_vertex = 0;
+    // adjacency symmetric (every u->v has a v->u of equal multiplicity):
+    // 1 / 0, or -1 when not checked (checked on upload when fully resident)
+    int symmetric = -1;
     asc::TravState* trav = nullptr;
     asc::PrState* pr = nullptr;
 };
@@ -206,4 +209,5 @@ void graph_free_device(asc_graph* g);
 void trav_free(TravState* t);
 void pr_free(PrState* p);
 int effective_heavy_threshold(const asc_graph* g);
+int check_symmetric(asc_graph* g);   // sets g->symmetric from the device copy
 }  // namespace asc

@@ -36,200 +36,13 @@
 #include "device_common.hpp"
 #include "internal.hpp"
 #include "traverse.hpp"
+#include "frontier.hpp"
 
 namespace asc {
 
 // ---------------------------------------------------------------------------
 // device code
 // ---------------------------------------------------------------------------
-
-enum { TO_NONE = 0, TO_RES = 1, TO_PEND = 2, TO_FAR = 3 };
-
-template <int ALGO>
-__device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
-    return ALGO == ALG_SSSP ? sv + w : sv;  // BFS passes level + 2 as sv
-}
-
-// Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
-// and routing of an improved vertex.
-template <int ALGO, class OffT, class EdgeT>
-__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
-    if (ALGO == ALG_BFS) {
-        const uint32_t bit = 1u << (v & 31);
-        uint32_t* word = a.visited + (v >> 5);
-        if (*word & bit) return TO_NONE;
-        if (atomicOr(word, bit) & bit) return TO_NONE;
-        a.val[v] = cand;
-        return v < a.vs ? TO_RES : TO_PEND;
-    } else {
-        if (cand >= a.val[v]) return TO_NONE;
-        const uint32_t old = atomicMin(a.val + v, cand);
-        if (old <= cand) return TO_NONE;
-        if (ALGO == ALG_SSSP && cand >= a.theta)
-            return atomicMax(a.fstamp + v, a.fepoch) < a.fepoch ? TO_FAR : TO_NONE;
-        if (v < a.vs) return atomicMax(a.stamp + v, level + 1) < level + 1 ? TO_RES : TO_NONE;
-        return atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
-    }
-}
-
-template <int ALGO>
-struct Pushes {
-    WaveStage<kStageCap> res, pend, far;
-
-    template <class A>
-    __device__ __forceinline__ void push(const A& a, Slot* nxt, int to, uint32_t v) {
-        res.push(to == TO_RES, v, a.q_out, &nxt->n_res);
-        pend.push(to == TO_PEND, v, a.pend_out, a.pend_count);
-        if (ALGO == ALG_SSSP) far.push(to == TO_FAR, v, a.far_out, a.far_count);
-    }
-    template <class A>
-    __device__ __forceinline__ void flush(const A& a, Slot* nxt) {
-        res.flush(a.q_out, &nxt->n_res);
-        pend.flush(a.pend_out, a.pend_count);
-        if (ALGO == ALG_SSSP) far.flush(a.far_out, a.far_count);
-    }
-};
-
-#define ASC_STAGE_DECL                                              \
-    __shared__ uint32_t s_stage[3][kWavesPerBlock][kStageCap];      \
-    Pushes<ALGO> pq{{s_stage[0][wave_id()], 0}, {s_stage[1][wave_id()], 0}, {s_stage[2][wave_id()], 0}}
-
-template <int ALGO, class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
-    __shared__ uint32_t s_pref[kBlock];
-    __shared__ OffT s_beg[kBlock];
-    __shared__ uint32_t s_sv[kBlock];
-    __shared__ uint32_t s_tmp[kWavesPerBlock];
-    ASC_STAGE_DECL;
-
-    Slot* cur = a.slots + (level & 3);
-    Slot* nxt = a.slots + ((level + 1) & 3);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        Slot* z = a.slots + ((level + 2) & 3);
-        z->n_res = 0;
-        z->heavy = 0;
-    }
-    const uint32_t n = cur->n_res;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    unsigned long long my_edges = 0, my_verts = 0;
-
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-        const uint32_t i = base + tid;
-        uint32_t deg = 0, sv = 0;
-        OffT beg = 0;
-        if (i < n) {
-            const uint32_t u = a.q_in[i];
-            beg = a.off[u];
-            uint64_t d = (uint64_t)(a.off[u + 1] - beg);
-            if (ALGO == ALG_BFS) {
-                sv = level + 2;
-            } else {
-                sv = a.val[u];
-                if (a.done[u] == sv) d = 0;  // already expanded at this value
-                else a.done[u] = sv;
-            }
-            if (d >= a.heavy_thr) {
-                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
-                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
-                a.heavy_u[idx] = u;
-                a.heavy_sv[idx] = sv;
-                a.heavy_base[idx] = p & kHeavyMask;
-            } else {
-                deg = (uint32_t)d;
-            }
-        }
-        uint32_t total;
-        const uint32_t pre = block_scan_excl(deg, s_tmp, total);
-        s_pref[tid] = pre;
-        s_beg[tid] = beg;
-        s_sv[tid] = sv;
-        __syncthreads();
-        const uint32_t cnt = min(n - base, (uint32_t)kBlock);
-        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
-            const uint32_t e = eb + lane;
-            int to = TO_NONE;
-            uint32_t v = 0;
-            if (e < total) {
-                uint32_t k = 0;
-#pragma unroll
-                for (uint32_t step = kBlock / 2; step > 0; step >>= 1)
-                    if (k + step < cnt && s_pref[k + step] <= e) k += step;
-                const EdgeT ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
-                v = edge_dst(ed);
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
-            }
-            pq.push(a, nxt, to, v);
-        }
-        my_edges += total;
-        my_verts += cnt;
-        __syncthreads();
-    }
-    pq.flush(a, nxt);
-    if (tid == 0 && my_verts) {
-        atomicAdd(&a.stats->edges_local, my_edges);
-        atomicAdd(&a.stats->verts_local, my_verts);
-    }
-}
-
-template <int ALGO, class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
-    constexpr int kWin = 8;  // a tile spans <= kTile / kMinHeavy + 1 heavy entries
-    __shared__ unsigned long long s_hb[kWin];
-    __shared__ OffT s_hoff[kWin];
-    __shared__ uint32_t s_hsv[kWin];
-    __shared__ uint32_t s_k0;
-    ASC_STAGE_DECL;
-
-    Slot* cur = a.slots + (level & 3);
-    Slot* nxt = a.slots + ((level + 1) & 3);
-    const unsigned long long h = cur->heavy;
-    const uint32_t nh = (uint32_t)(h >> kHeavyShift);
-    const uint64_t total = h & kHeavyMask;
-    if (nh == 0) return;
-    const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    unsigned long long my_edges = 0;
-
-    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
-        if (tid == 0) {
-            uint32_t lo = 0, hi = nh - 1;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
-            }
-            s_k0 = lo;
-        }
-        __syncthreads();
-        const uint32_t k0 = s_k0;
-        const uint32_t nwin = min(nh - k0, (uint32_t)kWin);
-        if (tid < nwin) {
-            s_hb[tid] = a.heavy_base[k0 + tid];
-            s_hoff[tid] = a.off[a.heavy_u[k0 + tid]];
-            s_hsv[tid] = a.heavy_sv[k0 + tid];
-        }
-        __syncthreads();
-        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
-        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
-            const uint32_t rr = r + lane;
-            int to = TO_NONE;
-            uint32_t v = 0;
-            if (rr < tlen) {
-                const uint64_t e = t0 + rr;
-                uint32_t k = 0;
-#pragma unroll
-                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
-                    if (j < nwin && s_hb[j] <= e) k = j;
-                const EdgeT ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
-                v = edge_dst(ed);
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
-            }
-            pq.push(a, nxt, to, v);
-        }
-        my_edges += tlen;
-        __syncthreads();
-    }
-    pq.flush(a, nxt);
-    if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
-}
 
 // Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
 // the streamed list (list[k] owns [prefix[k], prefix[k+1]), no zero-degree
@@ -437,6 +250,7 @@ void trav_free(TravState* t) {
     }
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
     f(t->od_list); f(t->od_prefix); f(t->slots); f(t->piles); f(t->stats);
+    f(t->bm[0]); f(t->bm[1]); f(t->dcnt);
     for (int i = 0; i < 2; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
         if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
@@ -446,7 +260,7 @@ void trav_free(TravState* t) {
     delete t;
 }
 
-static int trav_ensure(asc_graph* g) {
+int trav_ensure(asc_graph* g) {
     const uint64_t V = g->h.V;
     const size_t EB = g->packed ? 4 : g->h.edge_bytes();  // bytes per streamed edge
     TravState* t = g->trav;
@@ -503,27 +317,7 @@ static hipEvent_t pooled_event(TravState* t, size_t i) {
     return t->ev_pool[i];
 }
 
-static hipEvent_t next_event(TravState* t) { return pooled_event(t, t->ev_next++); }
-
-// Launches through `launch`; when profiling, brackets it with an event pair on
-// the launch stream and records the pair under `kind`.
-template <class F>
-static int timed_launch(TravState* t, bool prof, int kind, hipStream_t s, F&& launch) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (prof) {
-        a = next_event(t);
-        b = next_event(t);
-        if (!a || !b) return ASC_ERR_DEVICE;
-        ASC_HIP(hipEventRecord(a, s));
-    }
-    launch();
-    ASC_HIP(hipGetLastError());
-    if (prof) {
-        ASC_HIP(hipEventRecord(b, s));
-        t->timed.push_back({kind, a, b});
-    }
-    return ASC_OK;
-}
+hipEvent_t next_event(TravState* t) { return pooled_event(t, t->ev_next++); }
 
 int effective_heavy_threshold(const asc_graph* g) {
     uint64_t thr = g->heavy_threshold > 0 ? (uint64_t)g->heavy_threshold : 2048;
@@ -843,8 +637,17 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         last_split_empty = mode == M_SPLIT && nr == 0 && npend == 0;
     }
     const double t1 = now_ms();
+    return finish_traversal(g, ALGO, PATH_FRONTIER, out, st, rc, t1 - t0, level);
+}
+
+int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats* st, const RunCounters& rc,
+                     double total_ms, uint32_t iterations) {
+    asc_ctx* ctx = g->ctx;
+    TravState* t = g->trav;
+    const uint64_t V = g->h.V;
+    const bool prof = ctx->cfg.profile != 0;
     double static_ms = 0;
-    asc_kernel_stats kern[4];
+    asc_kernel_stats kern[KERN_COUNT];
     std::memset(kern, 0, sizeof(kern));
     for (auto& x : t->timed) {
         float ms = 0;
@@ -856,7 +659,7 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
             kern[x.kind].launches++;
         }
     }
-    if (prof) static_ms = kern[KERN_LOCAL].ms + kern[KERN_HEAVY].ms;
+    if (prof) static_ms = kern[KERN_LOCAL].ms + kern[KERN_HEAVY].ms + kern[KERN_BU].ms + kern[KERN_UF].ms;
     ASC_HIP(hipMemcpy(t->h_stats.p, t->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
     const DevStats& ds = t->h_stats[0];
     kern[KERN_LOCAL].edges = ds.edges_local;
@@ -864,25 +667,30 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     kern[KERN_HEAVY].edges = ds.edges_heavy;
     kern[KERN_PART].edges = ds.edges_part;
     kern[KERN_PART].vertices = ds.verts_part;
+    kern[KERN_BU].edges = ds.edges_bu;
+    kern[KERN_BU].vertices = ds.verts_bu;
+    kern[KERN_UF].edges = ds.edges_uf;
+    kern[KERN_UF].vertices = path == PATH_UNION_FIND ? V : 0;
     if (out) ASC_HIP(hipMemcpy(out, t->val, 4 * V, hipMemcpyDeviceToHost));
 
     if (st) {
         std::memset(st, 0, sizeof(*st));
-        st->total_ms = t1 - t0;
+        st->total_ms = total_ms;
         st->pre_ms = rc.pre_ms;
         st->static_ms = static_ms;
         st->kernel_ms = static_ms + kern[KERN_PART].ms;
         st->overload_ms = rc.overload_ms;
         st->fill_ms = rc.fill_ms;
-        st->iterations = level;
+        st->iterations = iterations;
         st->parts = rc.parts;
         st->node_sum = rc.node_sum;
-        st->static_edges_traversed = ds.edges_local + ds.edges_heavy;
-        st->edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_part;
+        st->static_edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_bu + ds.edges_uf;
+        st->edges_traversed = st->static_edges_traversed + ds.edges_part;
         st->streamed_edges = rc.streamed_edges;
         st->streamed_bytes = rc.streamed_edges * (g->packed ? 4 : g->h.edge_bytes());
-        st->vertices_processed = ds.verts_local;
+        st->vertices_processed = ds.verts_local + ds.verts_bu;
         st->host_cores = hardware_threads();
+        st->path = path;
         std::memcpy(st->kern, kern, sizeof(kern));
     }
     if (out) {
@@ -894,11 +702,11 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
         host_pool().parallel_for(V, [&](uint64_t b, uint64_t e) {
             uint64_t r = 0, d = 0;
             for (uint64_t v = b; v < e; v++) {
-                if (ALGO == ALG_CC || out[v] != unreached) {
+                if (algo == ALG_CC || out[v] != unreached) {
                     r++;
                     d += off[v + 1] - off[v];
                 }
-                if (ALGO != ALG_CC && conv == ASC_VALUES_ZERO_BASED)
+                if (algo != ALG_CC && conv == ASC_VALUES_ZERO_BASED)
                     out[v] = out[v] == unreached ? 0xFFFFFFFFu : out[v] - 1;
             }
             reached += r;
@@ -920,6 +728,10 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
 }
 
 int traverse(asc_graph* g, int algo, uint32_t src, uint32_t* out, asc_stats* st) {
+    const asc_config& cfg = g->ctx->cfg;
+    if (algo == ALG_BFS && cfg.bfs_direction == ASC_BFS_AUTO && resident_symmetric(g))
+        return bfs_direction_opt(g, src, out, st);
+    if (algo == ALG_CC && cfg.cc_method == ASC_CC_AUTO && resident_symmetric(g)) return cc_union_find(g, out, st);
     const bool w = g->h.kind == ASC_KIND_WCSR;
     const bool o4 = g->off_bytes == 4;
     switch (algo) {

@@ -41,10 +41,23 @@ struct DevStats {
     unsigned long long edges_heavy;   // k_expand_heavy
     unsigned long long edges_part;    // k_expand_part
     unsigned long long verts_part;
-    unsigned long long pad[3];
+    unsigned long long edges_bu;      // k_bfs_bottom_up: edge records read
+    unsigned long long verts_bu;      //   unvisited vertices scanned
+    unsigned long long edges_uf;      // union-find link passes: edge records read
 };
 
-enum { KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3 };
+enum { KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3, KERN_BU = 4, KERN_UF = 5, KERN_COUNT = 6 };
+
+// asc_stats.path
+enum { PATH_FRONTIER = 0, PATH_DIRECTION_OPT = 1, PATH_UNION_FIND = 2 };
+
+// Counters of the resident fast paths (direction-optimizing BFS, union-find CC).
+struct DoCounters {
+    unsigned long long n_found;       // vertices reached by a bottom-up step
+    unsigned long long m_found;       // their summed degrees
+    unsigned long long m_frontier;    // summed degrees of a top-down step's output
+    unsigned long long pad;
+};
 
 template <class OffT, class EdgeT>
 struct TravArgs {
@@ -120,6 +133,11 @@ struct TravState {
     HostBuf<uint32_t> h_od, h_list;
     HostBuf<unsigned long long> h_prefix;
     HostBuf<char> stage[2];
+    // resident fast paths: frontier bitmaps, counters, sampled labels
+    uint32_t* bm[2] = {nullptr, nullptr};
+    DoCounters* dcnt = nullptr;
+    HostBuf<DoCounters> h_dcnt;
+    HostBuf<uint32_t> h_sample;
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr}, ev_meta = nullptr;
     std::vector<hipEvent_t> ev_pool;
     // event pairs recorded during the current run: (kernel kind, start, end)
@@ -132,5 +150,39 @@ struct TravState {
 };
 
 int traverse(asc_graph* g, int algo, uint32_t src, uint32_t* out, asc_stats* st);
+int trav_ensure(asc_graph* g);
+hipEvent_t next_event(TravState* t);
+
+// Launches through `launch`; when profiling, brackets it with an event pair on
+// the launch stream and records the pair under `kind`.
+template <class F>
+int timed_launch(TravState* t, bool prof, int kind, hipStream_t s, F&& launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (prof) {
+        a = next_event(t);
+        b = next_event(t);
+        if (!a || !b) return ASC_ERR_DEVICE;
+        ASC_HIP(hipEventRecord(a, s));
+    }
+    launch();
+    ASC_HIP(hipGetLastError());
+    if (prof) {
+        ASC_HIP(hipEventRecord(b, s));
+        t->timed.push_back({kind, a, b});
+    }
+    return ASC_OK;
+}
+
+// Shared end of a traversal run: kernel statistics from the recorded events
+// and DevStats, the values copied to `out` (value convention applied), and
+// the reached count / degree sum.
+int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats* st, const RunCounters& rc,
+                     double total_ms, uint32_t iterations);
+
+// Fast paths for graphs whose edges are all resident and whose adjacency is
+// symmetric: direction-optimizing BFS and union-find CC.
+bool resident_symmetric(const asc_graph* g);
+int bfs_direction_opt(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st);
+int cc_union_find(asc_graph* g, uint32_t* out, asc_stats* st);
 
 }  // namespace asc

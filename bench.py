@@ -130,18 +130,80 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0):
     return out
 
 
-def _kernel_rate(st, name, s_e, s_o):
+def kernel_bytes(st, name, s_e, s_o, V=0):
+    """Algorithmic bytes of one kernel kind over a run.  pr_pull: per in-edge
+    the source id and its contribution, per vertex an offset pair, its
+    rank and contribution.  bottom_up: per edge read the id and one frontier
+    bitmap word, per scanned vertex an offset pair and its value, per launch
+    the visited and next bitmaps (V / 8 bytes each).  union_find: per edge
+    read the id and two parent probes, per vertex and link pass an offset
+    pair and its parent."""
+    k = st["kernels"][name]
+    if name == "pr_pull":
+        return k["edges"] * 8 + k["vertices"] * (2 * s_o + 16)
+    if name == "bottom_up":
+        return k["edges"] * (s_e + 4) + k["vertices"] * (2 * s_o + 4) + k["launches"] * V // 4
+    if name == "union_find":
+        return k["edges"] * (s_e + 8) + k["vertices"] * (2 * s_o + 4) * max(k["launches"], 1)
+    return algorithmic_bytes(st["kernels"], s_e, s_o)[name]
+
+
+def _kernel_rate(st, name, s_e, s_o, V=0):
     k = st["kernels"][name]
     if not k["ms"]:
         return None
-    if name == "pr_pull":
-        b = k["edges"] * 8 + k["vertices"] * (2 * s_o + 16)
-    else:
-        b = algorithmic_bytes(st["kernels"], s_e, s_o)[name]
-    return round(b / (k["ms"] / 1e3) / 1e9, 1)
+    return round(kernel_bytes(st, name, s_e, s_o, V) / (k["ms"] / 1e3) / 1e9, 1)
 
 
-def per_algo_bench(A, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
+def _kernel_table(st, s_e, s_o, V):
+    """{kernel: {ms, launches, GBps, frac}} for the kernels a run used."""
+    out = {}
+    for name, k in st["kernels"].items():
+        if k["ms"] <= 0:
+            continue
+        gbps = kernel_bytes(st, name, s_e, s_o, V) / (k["ms"] / 1e3) / 1e9
+        out[name] = {"ms": round(k["ms"], 3), "launches": k["launches"], "GBps": round(gbps, 1),
+                     "frac": round(gbps / HBM_PEAK_GBPS, 4)}
+    return out
+
+
+def resident_bench(A, _abi, h, steps):
+    """BFS, SSSP and CC on one fully resident weighted R-MAT graph (u64
+    offsets, {dst, w} records).  BFS runs direction-optimizing and CC by
+    union-find (the graph is symmetric); both are compared bit for bit with
+    the library's top-down BFS and label-propagation CC on the same device."""
+    off, ew = h.offsets(), h.edge_words()
+    V, E = h.num_vertices, h.num_edges
+    deg = np.diff(off)
+    src = int(np.argmax(deg))
+    out = {"V": V, "E_directed": E, "source": src}
+    s_e, s_o = 8, 8
+    with A.Context(profile=True) as ctx, ctx.from_arrays(off, ew) as g:
+        out["symmetric"] = g.info()["symmetric"]
+        for algo in ("bfs", "sssp", "cc"):
+            fn = getattr(g, algo)
+            args = () if algo == "cc" else (src,)
+            first, st = fn(*args)
+            runs = [fn(*args, output=False)[1] for _ in range(steps)]
+            ms = float(np.median([r["total_ms"] for r in runs]))
+            num = E / 2 if algo == "cc" else st["reached_degree_sum"] / 2
+            out[algo] = {"gteps": round(num / (ms / 1e3) / 1e9, 3), "ms": round(ms, 3), "path": st["path"],
+                         "iterations": st["iterations"], "reached": st["reached"],
+                         "kernels": _kernel_table(runs[len(runs) // 2], s_e, s_o, V)}
+            if algo in ("bfs", "cc"):
+                with A.Context(profile=True, bfs_direction=_abi.BFS_TOP_DOWN, cc_method=_abi.CC_LABEL_PROP) as c2, \
+                        c2.from_arrays(off, ew) as g2:
+                    ref, st2 = getattr(g2, algo)(*args)
+                    plain = [getattr(g2, algo)(*args, output=False)[1]["total_ms"] for _ in range(max(1, steps // 2))]
+                out[algo]["exact_vs_plain"] = bool(np.array_equal(first, ref))
+                out[algo]["plain_ms"] = round(float(np.median(plain)), 3)
+                out[algo]["plain_path"] = st2["path"]
+                del ref
+            del first
+    return out
+
+
+def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
     """GTEPS per algorithm: BFS on R-MAT (C1 shape) and CC / PageRank on an
     Erdos-Renyi graph (C3 shape), fully resident."""
     out = {}
@@ -162,8 +224,8 @@ def per_algo_bench(A, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
         tcpu = time.perf_counter() - tc
         out["bfs"] = {"workload": f"bfs_rmat_s{bfs_scale}_ef16_src0_u32off", "gteps": round(gteps, 3),
                       "ms": round(ms, 3), "levels": st["iterations"], "reached": st["reached"],
-                      "exact": bool(np.array_equal(lv, ref)),
-                      "expand_local_GBps": _kernel_rate(runs[-1], "expand_local", 4, 4),
+                      "exact": bool(np.array_equal(lv, ref)), "path": st["path"],
+                      "kernels": _kernel_table(runs[len(runs) // 2], 4, 4, h.num_vertices),
                       "cpu_port_gteps": round(st["reached_degree_sum"] / 2 / tcpu / 1e9, 3), "cpu_cores": cpu_threads}
         g.close()
     del h, off, dst
@@ -176,12 +238,19 @@ def per_algo_bench(A, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
     log(f"er graph: V={V} E={E} ({time.perf_counter() - t:.1f}s)")
     with A.Context(profile=True) as ctx:
         with ctx.from_arrays(off, dst) as g:
-            _, st = g.cc(output=False)
+            lab, st = g.cc()
             ccs = [g.cc(output=False)[1] for _ in range(max(1, steps // 2))]
             ms = float(np.median([c["total_ms"] for c in ccs]))
             out["cc"] = {"workload": f"cc_er_v2^{er_log_v}_e2^{er_log_e}", "gteps": round(E / 2 / (ms / 1e3) / 1e9, 3),
-                         "ms": round(ms, 3), "iterations": st["iterations"],
-                         "expand_local_GBps": _kernel_rate(ccs[-1], "expand_local", 4, 8)}
+                         "ms": round(ms, 3), "iterations": st["iterations"], "path": st["path"],
+                         "components": int(np.count_nonzero(lab == np.arange(V, dtype=np.uint32))),
+                         "kernels": _kernel_table(ccs[len(ccs) // 2], 4, 8, V)}
+        # label propagation on the same graph: the labels must be identical
+        with A.Context(profile=True, cc_method=_abi.CC_LABEL_PROP) as c2, c2.from_arrays(off, dst) as g2:
+            ref, st2 = g2.cc()
+            out["cc"]["exact_vs_plain"] = bool(np.array_equal(lab, ref))
+            out["cc"]["plain_ms"] = round(st2["total_ms"], 3)
+        del lab, ref
         outdeg = np.diff(off).astype(np.uint32)
         with ctx.from_arrays(off, dst, out_degree=outdeg) as g:
             _, res, _ = g.pagerank(iterations=20)
@@ -213,6 +282,8 @@ def main():
     ap.add_argument("--algo-er", type=str, default="26,30",
                     help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
     ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--resident-scale", type=int, default=26,
+                    help="R-MAT scale of the resident BFS/SSSP/CC section (0 = skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also launch one plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -358,9 +429,18 @@ def main():
     if args.algo_bfs_scale > 0 and args.algo_er:
         from oracle import oracle as O
         lv, le = (int(x) for x in args.algo_er.split(","))
-        result["per_algo"] = per_algo_bench(A, O, args.algo_bfs_scale, lv, le, args.steps,
+        result["per_algo"] = per_algo_bench(A, _abi, O, args.algo_bfs_scale, lv, le, args.steps,
                                             args.cpu_threads or min(16, os.cpu_count() or 1))
         result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
+    if args.resident_scale > 0:
+        t = time.perf_counter()
+        hr = A.rmat(args.resident_scale, 16, seed=args.seed, weighted=True, wmin=1, wmax=64)
+        log(f"resident graph: R-MAT s{args.resident_scale} ef16 weighted V={hr.num_vertices} E={hr.num_edges} "
+            f"({time.perf_counter() - t:.1f}s)")
+        result["resident"] = {"workload": f"rmat_s{args.resident_scale}_ef16_w1-64_resident_u64off",
+                              **resident_bench(A, _abi, hr, args.steps)}
+        hr.free()
+        del hr
     if args.serve_scale > 0:
         result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0])
     print(json.dumps(result), flush=True)

@@ -89,6 +89,24 @@ typedef enum {
                                host memory directly                         */
 } asc_stream_mode;
 
+/* BFS direction (asc_config.bfs_direction). */
+typedef enum {
+    ASC_BFS_AUTO = 0,        /* direction-optimizing (top-down / bottom-up
+                                switching) when every edge is resident and
+                                the adjacency is symmetric; else top-down    */
+    ASC_BFS_TOP_DOWN = 1     /* always top-down frontier expansion          */
+} asc_bfs_direction;
+
+/* CC method (asc_config.cc_method).  Both give every vertex the smallest
+ * vertex id that reaches it; on a symmetric graph that is the smallest id of
+ * its connected component. */
+typedef enum {
+    ASC_CC_AUTO = 0,         /* union-find (sampled hooking + compression)
+                                when every edge is resident and the adjacency
+                                is symmetric; else label propagation        */
+    ASC_CC_LABEL_PROP = 1    /* always min-label propagation                */
+} asc_cc_method;
+
 typedef struct {
     int device;                  /* HIP device ordinal                        */
     double advise_rate;          /* --adviseK: fraction of the device edge
@@ -117,6 +135,8 @@ typedef struct {
     int stream_pack;             /* 1 (default): stream weighted on-demand
                                     edges as one u32 word, dst | (w - wmin) <<
                                     bits(V-1), when both fit; 0: 8-byte records */
+    int bfs_direction;           /* asc_bfs_direction                         */
+    int cc_method;               /* asc_cc_method                             */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -173,6 +193,10 @@ typedef struct {
     uint64_t device_bytes;       /* device memory the graph holds             */
     uint32_t max_degree;
     uint32_t max_degree_vertex;
+    int symmetric;               /* 1: every u->v has a v->u of the same
+                                    multiplicity; 0: not; -1: not checked
+                                    (checked on upload when fully resident)  */
+    int pad;
 } asc_graph_info;
 int asc_graph_get_info(const asc_graph* g, asc_graph_info* info);
 
@@ -212,10 +236,14 @@ typedef struct {
     double kernel_ms;            /* sum of relax-kernel durations (events)    */
     double pr_residual;          /* PageRank: last iteration's L1 residual    */
     int host_cores;
-    /* Per relax kernel, filled when asc_config.profile = 1 (else zero):
+    int path;                    /* 0 frontier expansion, 1 direction-
+                                    optimizing BFS, 2 union-find CC          */
+    /* Per kernel kind; durations filled when asc_config.profile = 1, edge
+     * and vertex counts always:
      * [0] resident frontier expansion, [1] grid-cooperative heavy vertices,
-     * [2] on-demand parts, [3] PageRank pull. */
-    asc_kernel_stats kern[4];
+     * [2] on-demand parts, [3] PageRank pull, [4] BFS bottom-up steps,
+     * [5] union-find link passes. */
+    asc_kernel_stats kern[6];
 } asc_stats;
 
 /* ---- algorithms (CalculateOpt.cu:14-721) ------------------------------- */
