@@ -237,20 +237,33 @@ __global__ __launch_bounds__(kBlock) void k_queue_to_bitmap(const uint32_t* q, c
     }
 }
 
-// Bitmap -> frontier queue; *count (zero beforehand) gets the entries.
+// Bitmap -> frontier queue; *count (zero beforehand) gets the entries.  A
+// thread owns one 32-vertex word; a workgroup scans its 256 words' popcounts
+// and reserves their queue space with a single atomic.
 __global__ __launch_bounds__(kBlock) void k_bitmap_to_queue(const uint32_t* bm, uint32_t V, uint32_t* q,
                                                             uint32_t* count) {
-    const uint32_t lane = lane_id();
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
-    for (uint64_t wv = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); wv * kWave < V; wv += nwaves) {
-        const uint32_t v = (uint32_t)(wv * kWave) + lane;
-        const bool on = v < V && ((bm[v >> 5] >> (v & 31)) & 1u);
-        const uint64_t m = wave_ballot(on);
-        if (m == 0) continue;
-        uint32_t at = 0;
-        if (lane == 0) at = atomicAdd(count, (uint32_t)__popcll(m));
-        at = __shfl(at, 0, kWave);
-        if (on) q[at + lanes_below(m)] = v;
+    __shared__ uint32_t s_scan[kWavesPerBlock];
+    __shared__ uint32_t s_base;
+    const uint32_t words = (V + 31) / 32;
+    for (uint32_t w0 = blockIdx.x * kBlock; w0 < words; w0 += gridDim.x * kBlock) {
+        const uint32_t wi = w0 + threadIdx.x;
+        uint32_t word = 0;
+        if (wi < words) {
+            word = bm[wi];
+            const uint32_t valid = V - wi * 32;
+            if (valid < 32) word &= (1u << valid) - 1u;
+        }
+        uint32_t total;
+        const uint32_t pre = block_scan_excl((uint32_t)__popc(word), s_scan, total);
+        if (threadIdx.x == 0) s_base = total ? atomicAdd(count, total) : 0;
+        __syncthreads();
+        uint32_t at = s_base + pre;
+        while (word) {
+            const uint32_t b = __ffs(word) - 1;
+            q[at++] = wi * 32 + b;
+            word &= word - 1;
+        }
+        __syncthreads();
     }
 }
 
@@ -358,7 +371,8 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
         } else if (bottom_up && nf < V / kBeta && nf < prev_nf) {
             // bitmap -> queue, slot ring restarted at this level
             ASC_HIP(hipMemsetAsync(t->slots, 0, sizeof(Slot) * 4, s));
-            hipLaunchKernelGGL(k_bitmap_to_queue, dim3(bu_grid), dim3(kBlock), 0, s, t->bm[cur], (uint32_t)V,
+            hipLaunchKernelGGL(k_bitmap_to_queue, dim3(std::max<uint64_t>(1, std::min<uint64_t>((words + kBlock - 1) / kBlock, grid))),
+                               dim3(kBlock), 0, s, t->bm[cur], (uint32_t)V,
                                t->q[level & 1], &t->slots[level & 3].n_res);
             bottom_up = false;
         }
@@ -403,6 +417,10 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             mf = t->h_dcnt[0].m_frontier;
         }
         m_unexplored = m_unexplored > mf ? m_unexplored - mf : 0;
+        if (ctx->cfg.verbose > 1)
+            printf("bfs level %u %s: next frontier %llu vertices, %llu edges (%.3f ms)\n", level,
+                   bottom_up ? "bottom-up" : "top-down", (unsigned long long)nf, (unsigned long long)mf,
+                   now_ms() - t0);
         level++;
     }
     const double t1 = now_ms();
