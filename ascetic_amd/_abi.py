@@ -54,6 +54,7 @@ class Config(C.Structure):
         ("stream_pack", C.c_int),
         ("bfs_direction", C.c_int),
         ("cc_method", C.c_int),
+        ("dense_threshold", C.c_double),
     ]
 
 
@@ -117,6 +118,7 @@ class Stats(C.Structure):
         ("static_edges_traversed", C.c_uint64),
         ("streamed_edges", C.c_uint64),
         ("streamed_bytes", C.c_uint64),
+        ("dense_edges", C.c_uint64),
         ("vertices_processed", C.c_uint64),
         ("reached", C.c_uint64),
         ("reached_degree_sum", C.c_uint64),

@@ -131,6 +131,7 @@ void asc_config_init(asc_config* c) {
     c->stream_pack = 1;
     c->bfs_direction = ASC_BFS_AUTO;
     c->cc_method = ASC_CC_AUTO;
+    c->dense_threshold = 0.0;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

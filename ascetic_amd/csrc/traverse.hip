@@ -123,6 +123,16 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
 // Compacts a pile before it is streamed: drops zero-degree vertices and (SSSP,
 // CC) vertices already expanded at their current value, and snapshots the
 // value each kept vertex will be expanded with.
+// Chunk of on-demand vertex v: the largest i < n_ch with ch_v[i] <= v.
+__device__ __forceinline__ uint32_t chunk_of(const uint32_t* ch_v, uint32_t n_ch, uint32_t v) {
+    uint32_t lo = 0, hi = n_ch - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (ch_v[mid] <= v) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
 template <int ALGO, class OffT>
 __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const uint32_t* pile, uint32_t n,
                                                         uint32_t* val, uint32_t* done, uint32_t* out,
@@ -154,6 +164,87 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const u
             out_deg[pos] = deg;
             if (ALGO != ALG_BFS) out_sv[pos] = d;
         }
+    }
+}
+
+// Active edges and vertices of a pile per dense-copy chunk (the vertices
+// k_filter_pile keeps: degree > 0 and, for SSSP/CC, not yet expanded at their
+// value).
+template <int ALGO, class OffT>
+__global__ __launch_bounds__(kBlock) void k_pile_chunks(const OffT* off, const uint32_t* pile, uint32_t n,
+                                                        const uint32_t* val, const uint32_t* done,
+                                                        const uint32_t* ch_v, uint32_t n_ch,
+                                                        unsigned long long* ch_edges, uint32_t* ch_count) {
+    __shared__ unsigned long long s_h[kMaxChunks];
+    __shared__ uint32_t s_c[kMaxChunks];
+    for (uint32_t c = threadIdx.x; c < n_ch; c += kBlock) {
+        s_h[c] = 0;
+        s_c[c] = 0;
+    }
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const uint32_t v = pile[i];
+        const uint64_t d = (uint64_t)(off[v + 1] - off[v]);
+        if (d == 0) continue;
+        if (ALGO != ALG_BFS && val[v] == done[v]) continue;
+        const uint32_t c = chunk_of(ch_v, n_ch, v);
+        atomicAdd(&s_h[c], (unsigned long long)d);
+        atomicAdd(&s_c[c], 1u);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < n_ch; c += kBlock)
+        if (s_c[c]) {
+            atomicAdd(ch_edges + c, s_h[c]);
+            atomicAdd(ch_count + c, s_c[c]);
+        }
+}
+
+// k_filter_pile for a round with dense chunks: a kept vertex of a dense chunk
+// c goes to that chunk's list segment (cursor[c], placed by the host after
+// the sparse segment), every other kept vertex to the sparse segment
+// (cursor[n_ch], from 0).  Workgroups count their entries per segment in LDS
+// and reserve each segment's share with one atomic.
+template <int ALGO, class OffT>
+__global__ __launch_bounds__(kBlock) void k_filter_pile_chunked(const OffT* off, const uint32_t* pile, uint32_t n,
+                                                                uint32_t* val, uint32_t* done, uint32_t* out,
+                                                                uint32_t* out_sv, uint32_t* out_deg,
+                                                                const uint32_t* ch_v, uint32_t n_ch,
+                                                                const uint8_t* ch_dense, uint32_t* cursor) {
+    __shared__ uint32_t s_cnt[kMaxChunks + 1];
+    __shared__ uint32_t s_base[kMaxChunks + 1];
+    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+        for (uint32_t c = threadIdx.x; c <= n_ch; c += kBlock) s_cnt[c] = 0;
+        __syncthreads();
+        const uint32_t i = base + threadIdx.x;
+        bool keep = false;
+        uint32_t v = 0, d = 0, deg = 0, seg = 0, rank = 0;
+        if (i < n) {
+            v = pile[i];
+            const uint64_t dd = (uint64_t)(off[v + 1] - off[v]);
+            deg = dd > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dd;
+            keep = dd > 0;
+            if (ALGO != ALG_BFS && keep) {
+                d = val[v];
+                keep = done[v] != d;
+                if (keep) done[v] = d;
+            }
+            if (keep) {
+                const uint32_t c = chunk_of(ch_v, n_ch, v);
+                seg = ch_dense[c] ? c : n_ch;
+                rank = atomicAdd(&s_cnt[seg], 1u);
+            }
+        }
+        __syncthreads();
+        for (uint32_t c = threadIdx.x; c <= n_ch; c += kBlock)
+            if (s_cnt[c]) s_base[c] = atomicAdd(cursor + c, s_cnt[c]);
+        __syncthreads();
+        if (keep) {
+            const uint32_t pos = s_base[seg] + rank;
+            out[pos] = v;
+            out_deg[pos] = deg;
+            if (ALGO != ALG_BFS) out_sv[pos] = d;
+        }
+        __syncthreads();
     }
 }
 
@@ -250,14 +341,58 @@ void trav_free(TravState* t) {
     }
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
     f(t->od_list); f(t->od_prefix); f(t->slots); f(t->piles); f(t->stats);
-    f(t->bm[0]); f(t->bm[1]); f(t->dcnt);
+    f(t->bm[0]); f(t->bm[1]); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
+    f(t->d_cursor);
     for (int i = 0; i < 2; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
         if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
+        if (t->ev_stage[i]) (void)hipEventDestroy(t->ev_stage[i]);
     }
     if (t->ev_meta) (void)hipEventDestroy(t->ev_meta);
     for (auto e : t->ev_pool) (void)hipEventDestroy(e);
     delete t;
+}
+
+// Vertex-aligned chunks of the on-demand range for whole-chunk copies: each
+// holds at most one part of edges (a vertex larger than a part gets a chunk
+// of its own, never copied whole).  Needs the tail in pinned host memory
+// (packed words or the pinned copy) and gather streaming.
+static int plan_dense_chunks(asc_graph* g, TravState* t) {
+    t->ch_v.clear();
+    t->ch_e.clear();
+    const asc_config& cfg = g->ctx->cfg;
+    if (cfg.stream_mode != ASC_STREAM_GATHER || cfg.dense_threshold > 1.0) return ASC_OK;
+    if (!g->packed && !g->zc_tail.p) return ASC_OK;
+    const uint64_t V = g->h.V, vs = g->static_vertices;
+    const uint64_t* off = g->h.off.p;
+    const uint64_t cap = t->part_edges;
+    std::vector<uint32_t> cv;
+    std::vector<uint64_t> ce;
+    uint64_t v = vs;
+    while (v < V) {
+        uint64_t v1 = v + 1;
+        while (v1 < V && off[v1 + 1] - off[v] <= cap) v1++;
+        cv.push_back((uint32_t)v);
+        ce.push_back(off[v]);
+        v = v1;
+        if (cv.size() > kMaxChunks) return ASC_OK;   // too many: gather only
+    }
+    cv.push_back((uint32_t)V);
+    ce.push_back(off[V]);
+    const size_t n = cv.size() - 1;
+    ASC_HIP(hipMalloc(&t->d_ch_v, 4 * cv.size()));
+    ASC_HIP(hipMemcpy(t->d_ch_v, cv.data(), 4 * cv.size(), hipMemcpyHostToDevice));
+    ASC_HIP(hipMalloc(&t->d_ch_edges, 8 * n));
+    ASC_HIP(hipMalloc(&t->d_ch_count, 4 * n));
+    ASC_HIP(hipMalloc(&t->d_ch_dense, n));
+    ASC_HIP(hipMalloc(&t->d_cursor, 4 * (n + 1)));
+    ASC_TRY(t->h_ch_edges.alloc(n, true));
+    ASC_TRY(t->h_ch_count.alloc(n, true));
+    ASC_TRY(t->h_ch_dense.alloc(n, true));
+    ASC_TRY(t->h_cursor.alloc(n + 1, true));
+    t->ch_v = std::move(cv);
+    t->ch_e = std::move(ce);
+    return ASC_OK;
 }
 
 int trav_ensure(asc_graph* g) {
@@ -302,8 +437,12 @@ int trav_ensure(asc_graph* g) {
             }
             ASC_HIP(hipEventCreateWithFlags(&t->ev_copy[i], hipEventDisableTiming));
             ASC_HIP(hipEventCreateWithFlags(&t->ev_kern[i], hipEventDisableTiming));
+            ASC_HIP(hipEventCreateWithFlags(&t->ev_stage[i], hipEventDisableTiming));
+            // recorded once so that every copy can wait for its buffer's last reader
+            ASC_HIP(hipEventRecord(t->ev_kern[i], g->ctx->s_main));
         }
         ASC_HIP(hipEventCreateWithFlags(&t->ev_meta, hipEventDisableTiming));
+        ASC_TRY(plan_dense_chunks(g, t));
     }
     return ASC_OK;
 }
@@ -361,6 +500,10 @@ static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long 
 
 // Streams one pile: filter on the device, list to the host, u64 prefix of
 // degrees, then parts (gather + double-buffered copies) or zero-copy reads.
+// With dense chunks planned, the pile's active edges are first counted per
+// chunk; a chunk whose active edges reach asc_config.dense_threshold of its
+// edges is copied whole by DMA from the pinned tail (no host gather) and its
+// active vertices, kept in their own list segment, are expanded from it.
 template <int ALGO, class OffT, class EdgeT>
 static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, const uint32_t* pile, uint32_t npile,
                        int grid, RunCounters& rc) {
@@ -372,23 +515,69 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     const uint64_t nod = g->h.V - g->static_vertices;
     uint32_t* list_sv = t->od_list + nod;
     uint32_t* list_deg = t->od_list + 2 * nod;
-    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
-    hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096)),
-                       dim3(kBlock), 0, s, a.off, pile, npile, a.val, a.done, t->od_list, list_sv, list_deg,
-                       &t->piles->n_list);
-    ASC_HIP(hipGetLastError());
-    // count, kept list and degrees (computed on the device) in one round trip:
-    // the kept count is at most the pile size, so copy that many entries
-    uint32_t* h_deg = t->h_list.p + nod;
-    ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
-    ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
-    ASC_HIP(hipMemcpyAsync(h_deg, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
-    ASC_HIP(hipStreamSynchronize(s));
-    const uint32_t m = t->h_piles[0].n_list;
+    const bool pk = g->packed;
+    const size_t EB = pk ? 4 : g->h.edge_bytes();
+    const uint64_t S = g->static_edges;
+    const int pile_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096));
+
+    // active edges per chunk -> which chunks are copied whole
+    const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
+    uint32_t n_dense = 0, n_sparse = 0, m = 0;
+    if (n_ch) {
+        ASC_HIP(hipMemsetAsync(t->d_ch_edges, 0, 8 * (size_t)n_ch, s));
+        ASC_HIP(hipMemsetAsync(t->d_ch_count, 0, 4 * (size_t)n_ch, s));
+        hipLaunchKernelGGL((k_pile_chunks<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile, npile, a.val,
+                           a.done, t->d_ch_v, n_ch, t->d_ch_edges, t->d_ch_count);
+        ASC_HIP(hipGetLastError());
+        ASC_HIP(hipMemcpyAsync(t->h_ch_edges.p, t->d_ch_edges, 8 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipMemcpyAsync(t->h_ch_count.p, t->d_ch_count, 4 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipStreamSynchronize(s));
+        const double thr = ctx->cfg.dense_threshold > 0 ? ctx->cfg.dense_threshold : 0.5;
+        for (uint32_t c = 0; c < n_ch; c++) {
+            const uint64_t ce = t->ch_e[c + 1] - t->ch_e[c];
+            const bool d = t->h_ch_count[c] > 0 && ce <= t->part_edges && (double)t->h_ch_edges[c] >= thr * (double)ce;
+            t->h_ch_dense[c] = d ? 1 : 0;
+            if (d) n_dense++;
+            else n_sparse += t->h_ch_count[c];
+        }
+    }
+    if (n_dense) {
+        // segments: sparse vertices first, then one per dense chunk
+        uint32_t at = n_sparse;
+        t->h_cursor[n_ch] = 0;
+        for (uint32_t c = 0; c < n_ch; c++) {
+            t->h_cursor[c] = at;
+            if (t->h_ch_dense[c]) at += t->h_ch_count[c];
+        }
+        m = at;
+        ASC_HIP(hipMemcpyAsync(t->d_ch_dense, t->h_ch_dense.p, n_ch, hipMemcpyHostToDevice, s));
+        ASC_HIP(hipMemcpyAsync(t->d_cursor, t->h_cursor.p, 4 * ((size_t)n_ch + 1), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL((k_filter_pile_chunked<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile,
+                           npile, a.val, a.done, t->od_list, list_sv, list_deg, t->d_ch_v, n_ch,
+                           (const uint8_t*)t->d_ch_dense, t->d_cursor);
+        ASC_HIP(hipGetLastError());
+        ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipStreamSynchronize(s));
+    } else {
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
+        hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile, npile,
+                           a.val, a.done, t->od_list, list_sv, list_deg, &t->piles->n_list);
+        ASC_HIP(hipGetLastError());
+        // count, kept list and degrees (computed on the device) in one round
+        // trip: the kept count is at most the pile size, so copy that many
+        ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipStreamSynchronize(s));
+        m = t->h_piles[0].n_list;
+        n_sparse = m;
+    }
     if (m == 0) {
         rc.pre_ms += now_ms() - t_begin;
         return ASC_OK;
     }
+    const uint32_t* h_deg = t->h_list.p + nod;
     // u64 exclusive prefix of the degrees, in parallel
     const int nt = ctx->pool->size();
     std::vector<uint64_t> esum(nt + 1, 0);
@@ -408,7 +597,8 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     });
     const uint64_t T = esum[nt];
     t->h_prefix[m] = T;
-    rc.pre_ms += now_ms() - t_begin;
+    const double t_pre = now_ms();
+    rc.pre_ms += t_pre - t_begin;
     rc.node_sum += m;
     rc.stream_rounds++;
     ASC_HIP(hipMemcpyAsync(t->od_prefix, t->h_prefix.p, 8 * ((size_t)m + 1), hipMemcpyHostToDevice, s));
@@ -417,12 +607,10 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
     p.list = t->od_list;
     p.prefix = t->od_prefix;
     p.n = m;
-    p.tail_base = g->static_edges;
+    p.tail_base = S;
     p.pk_shift = g->pk_shift;
-    p.pk_mask = g->packed ? (1u << g->pk_shift) - 1 : 0xFFFFFFFFu;
+    p.pk_mask = pk ? (1u << g->pk_shift) - 1 : 0xFFFFFFFFu;
     p.pk_wmin = g->pk_wmin;
-    const bool pk = g->packed;
-    const size_t EB = pk ? 4 : g->h.edge_bytes();
     auto launch_part = [&](bool zc) {
         const int pgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (p.hi - p.lo + kTile - 1) / kTile));
         return timed_launch(t, prof, KERN_PART, s, [&] {
@@ -441,6 +629,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
                                    a, level, p, list_sv);
         });
     };
+    uint64_t dense_edges = 0;
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
         void* dptr = nullptr;
         ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
@@ -450,31 +639,65 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
         p.hi = T;
         ASC_TRY(launch_part(true));
         rc.parts += 1;
+        rc.streamed_edges += T;
     } else {
+        uint32_t nb = 0;   // device part buffers are used round-robin by both loops
+        // dense chunks: whole-chunk DMA from the pinned tail, then the chunk's
+        // list segment expanded with edges addressed from the chunk start
+        const char* tail = pk ? (const char*)g->ptail.p : (const char*)g->zc_tail.p;
+        for (uint32_t c = 0; c < n_ch && n_dense; c++) {
+            if (!t->h_ch_dense[c]) continue;
+            const int b = (int)(nb++ & 1);
+            const uint64_t e0 = t->ch_e[c], e1 = t->ch_e[c + 1];
+            ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
+            ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
+                                   ctx->s_copy));
+            ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+            ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
+            p.zc = (const EdgeT*)t->part[b];
+            p.zc32 = (const uint32_t*)t->part[b];
+            p.tail_base = e0;
+            p.lo = t->h_prefix[t->h_cursor[c]];
+            p.hi = t->h_prefix[t->h_cursor[c] + t->h_ch_count[c]];
+            if (p.hi > p.lo) ASC_TRY(launch_part(true));
+            ASC_HIP(hipEventRecord(t->ev_kern[b], s));
+            dense_edges += e1 - e0;
+        }
+        // the rest: host gather into pinned staging, double-buffered copies
+        const uint64_t Ts = t->h_prefix[n_sparse];
         const uint64_t PE = t->part_edges;
-        const uint64_t np = (T + PE - 1) / PE;
+        const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
-            const int b = (int)(q & 1);
-            const uint64_t lo = q * PE, hi = std::min(T, lo + PE);
-            if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_copy[b]));  // staging b drained
+            const int sb = (int)(q & 1);
+            const int b = (int)(nb++ & 1);
+            const uint64_t lo = q * PE, hi = std::min(Ts, lo + PE);
+            if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
-            gather_part(g, t->h_list.p, t->h_prefix.p, m, lo, hi, t->stage[b].p);
+            gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p);
             rc.fill_ms += now_ms() - tg;
-            if (q >= 2) ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
-            ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[b].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
+            ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
+            ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[sb].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
+            ASC_HIP(hipEventRecord(t->ev_stage[sb], ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.part = (const EdgeT*)t->part[b];
             p.part32 = (const uint32_t*)t->part[b];
+            p.tail_base = S;
             p.lo = lo;
             p.hi = hi;
             ASC_TRY(launch_part(false));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
         }
-        rc.parts += (uint32_t)np;
+        rc.parts += nb;
+        rc.streamed_edges += Ts + dense_edges;
+        rc.dense_edges += dense_edges;
     }
-    rc.streamed_edges += T;
     rc.overload_ms += now_ms() - t_begin;
+    if (ctx->cfg.verbose > 1)
+        printf("stream round %u: pile %u, %u dense chunks %llu edges, gathered %u vertices %llu edges, "
+               "%.3f ms (prepare %.3f)\n",
+               rc.stream_rounds, npile, n_dense, (unsigned long long)dense_edges, n_sparse,
+               (unsigned long long)t->h_prefix[n_sparse], now_ms() - t_begin, t_pre - t_begin);
     return ASC_OK;
 }
 
@@ -688,6 +911,7 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
         st->edges_traversed = st->static_edges_traversed + ds.edges_part;
         st->streamed_edges = rc.streamed_edges;
         st->streamed_bytes = rc.streamed_edges * (g->packed ? 4 : g->h.edge_bytes());
+        st->dense_edges = rc.dense_edges;
         st->vertices_processed = ds.verts_local + ds.verts_bu;
         st->host_cores = hardware_threads();
         st->path = path;

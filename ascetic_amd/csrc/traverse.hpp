@@ -16,6 +16,7 @@ constexpr int kTile = 1024;                     // edges per tile in edge-balanc
 constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
 constexpr int kBlocksPerCU = 8;
 constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
+constexpr uint32_t kMaxChunks = 1024;           // dense on-demand chunks (LDS histogram size)
 
 // Per-level counters, in a ring of four: level L reads slot L%4, appends to
 // slot (L+1)%4 and clears slot (L+2)%4.
@@ -102,8 +103,8 @@ struct PartArgs {
 
 struct RunCounters {
     double pre_ms = 0, overload_ms = 0, fill_ms = 0;
-    uint64_t node_sum = 0, streamed_edges = 0;
-    uint32_t parts = 0, stream_rounds = 0, splits = 0;
+    uint64_t node_sum = 0, streamed_edges = 0, dense_edges = 0;
+    uint32_t parts = 0, stream_rounds = 0, splits = 0, dense_chunks = 0;
 };
 
 struct TravState {
@@ -133,12 +134,27 @@ struct TravState {
     HostBuf<uint32_t> h_od, h_list;
     HostBuf<unsigned long long> h_prefix;
     HostBuf<char> stage[2];
+    // dense on-demand chunks: vertex ranges [ch_v[i], ch_v[i+1]) whose edges
+    // [ch_e[i], ch_e[i+1]) fit one part buffer (empty: dense copies off)
+    std::vector<uint32_t> ch_v;
+    std::vector<uint64_t> ch_e;
+    uint32_t* d_ch_v = nullptr;
+    unsigned long long* d_ch_edges = nullptr;   // active edges per chunk, this round
+    uint32_t* d_ch_count = nullptr;             // active vertices per chunk, this round
+    uint8_t* d_ch_dense = nullptr;              // chunk copied whole this round
+    uint32_t* d_cursor = nullptr;               // list segment cursors (n_ch + 1)
+    HostBuf<unsigned long long> h_ch_edges;
+    HostBuf<uint32_t> h_ch_count, h_cursor;
+    HostBuf<uint8_t> h_ch_dense;
     // resident fast paths: frontier bitmaps, counters, sampled labels
     uint32_t* bm[2] = {nullptr, nullptr};
     DoCounters* dcnt = nullptr;
     HostBuf<DoCounters> h_dcnt;
     HostBuf<uint32_t> h_sample;
+    // device part buffer b: copy into it done / last kernel reading it done;
+    // staging buffer i: copy out of it done
     hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr}, ev_meta = nullptr;
+    hipEvent_t ev_stage[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     // event pairs recorded during the current run: (kernel kind, start, end)
     struct Timed {

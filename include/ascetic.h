@@ -137,6 +137,12 @@ typedef struct {
                                     bits(V-1), when both fit; 0: 8-byte records */
     int bfs_direction;           /* asc_bfs_direction                         */
     int cc_method;               /* asc_cc_method                             */
+    double dense_threshold;      /* gather streaming: an on-demand chunk (a
+                                    vertex range of at most one part) whose
+                                    active edges reach this fraction of its
+                                    edges is copied whole by DMA from the
+                                    pinned tail instead of gathered; 0 = 0.5,
+                                    > 1 = never                              */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -230,6 +236,8 @@ typedef struct {
     uint64_t static_edges_traversed;
     uint64_t streamed_edges;     /* on-demand edges shipped over the link     */
     uint64_t streamed_bytes;
+    uint64_t dense_edges;        /* of streamed_edges: copied as whole chunks
+                                    by DMA (asc_config.dense_threshold)       */
     uint64_t vertices_processed; /* frontier entries expanded                 */
     uint64_t reached;            /* vertices with a final value != unreached  */
     uint64_t reached_degree_sum; /* sum of out-degrees of reached vertices    */

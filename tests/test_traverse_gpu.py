@@ -104,6 +104,44 @@ def test_streaming_matches_resident(name, mk, mode):
                     assert np.array_equal(lv, O.bfs_queue(csr, s))
 
 
+@pytest.mark.parametrize("thr", [1e-9, 0.5, 2.0])
+@pytest.mark.parametrize("name,mk", CASES[3:])
+def test_dense_chunk_copies(name, mk, thr):
+    """On-demand chunks copied whole by DMA (threshold ~0: every active chunk;
+    0.5: the default; 2: never) give the same values as the oracle, in the
+    4-byte packed, 8-byte and unweighted layouts."""
+    say("dense", name, "threshold", thr)
+    off, dst, _ = mk()
+    w = weights_for(off, 6)
+    csr = O.CSR(off, dst, w)
+    E = int(off[-1])
+    srcs = (0, int(np.argmax(np.diff(off))))
+    dense = 0
+    with A.Context(edge_budget_bytes=4 * (E // 3), part_bytes=4 * 4096, dense_threshold=thr) as ctx:
+        with make_graph(ctx, off, dst) as g:
+            for s in srcs:
+                lv, st = g.bfs(s)
+                assert np.array_equal(lv, O.bfs_queue(csr, s)), s
+                assert st["dense_edges"] <= st["streamed_edges"]
+                dense += st["dense_edges"]
+            lab, st = g.cc()
+            assert np.array_equal(lab, O.cc_minreach(csr))
+            dense += st["dense_edges"]
+    for pack in (True, False):
+        with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_pack=pack,
+                       dense_threshold=thr) as ctx:
+            with make_graph(ctx, off, dst, w) as g:
+                for s in srcs:
+                    d, st = g.sssp(s)
+                    assert np.array_equal(d, O.dijkstra(csr, s)), (pack, s)
+                    assert st["streamed_bytes"] == st["streamed_edges"] * (4 if pack else 8)
+                    dense += st["dense_edges"]
+    if thr > 1:
+        assert dense == 0
+    elif thr < 1e-6:
+        assert dense > 0
+
+
 def test_everything_on_demand(gpu_ctx):
     off, dst, _ = G.random_graph(3000, 12000, 9, symmetric=False)
     w = weights_for(off, 9)
