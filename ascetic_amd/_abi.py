@@ -26,7 +26,7 @@ ERR_STATE = -10
 FMT_AUTO, FMT_BCSR, FMT_BWCSR, FMT_BCSC, FMT_LEGACY32, FMT_LEGACY32W = range(6)
 KIND_CSR, KIND_WCSR, KIND_CSC = range(3)
 VALUES_REFERENCE, VALUES_ZERO_BASED = 0, 1
-STREAM_GATHER, STREAM_ZEROCOPY = 0, 1
+STREAM_GATHER, STREAM_ZEROCOPY, STREAM_HYBRID = 0, 1, 2
 PR_NORMALISED, PR_REFERENCE = 0, 1
 BFS_AUTO, BFS_TOP_DOWN = 0, 1
 CC_AUTO, CC_LABEL_PROP = 0, 1

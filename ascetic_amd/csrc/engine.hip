@@ -124,7 +124,8 @@ int graph_upload(asc_graph* g) {
     // PageRank streams whole vertex-range chunks of them by DMA every iteration;
     // gather streaming copies dense chunks whole by DMA, which needs pinned memory
     const bool pr_tail = g->h.kind == ASC_KIND_CSC;
-    const bool dense = ctx->cfg.stream_mode == ASC_STREAM_GATHER && ctx->cfg.dense_threshold <= 1.0;
+    const bool dense = ctx->cfg.stream_mode == ASC_STREAM_HYBRID ||
+                       (ctx->cfg.stream_mode == ASC_STREAM_GATHER && ctx->cfg.dense_threshold <= 1.0);
     if ((ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || pr_tail || dense) && g->static_edges < E && !g->packed) {
         const uint64_t words = (E - g->static_edges) * g->h.edge_words();
         ASC_TRY(g->zc_tail.alloc(words, true));

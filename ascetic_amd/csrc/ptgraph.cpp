@@ -7,7 +7,7 @@
 // accepted and ignored.  Additions:
 //   --budget <bytes>        device edge budget (replaces --adviseK)
 //   --part-mb <n>           on-demand part buffer size
-//   --stream gather|zerocopy
+//   --stream gather|zerocopy|hybrid
 //   --format auto|bcsr|bwcsr|bcsc|legacy32|legacy32w
 //   --values ref|zero       output convention (source = 1 / 0)
 //   --pr-mode ref|norm      PageRank: reference active-set FP64 (default) or
@@ -51,7 +51,7 @@ struct Args {
 void usage() {
     fprintf(stderr,
             "usage: ptgraph --input <graph file> [--type bfs|cc|sssp|pr|info] [--source N] [--adviseK f]\n"
-            "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy] [--format fmt]\n"
+            "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy|hybrid] [--format fmt]\n"
             "               [--values ref|zero] [--pr-mode ref|norm] [--iters n] [--out file] [--verbose 0|1]\n"
             "       ptgraph --type gen --gen rmat|er [--scale s --ef f | --vertices n --samples m] [--seed x]\n"
             "               [--weighted 0|1] [--wmax w] [--csc 0|1] --format fmt --out file\n");
@@ -153,7 +153,8 @@ int main(int argc, char** argv) {
     cfg.advise_rate = a.num("--adviseK", 0.0);
     cfg.edge_budget_bytes = (uint64_t)a.num("--budget", 0);
     cfg.part_bytes = (uint64_t)a.num("--part-mb", 0) * (1ull << 20);
-    cfg.stream_mode = a.get("--stream", "gather") == "zerocopy" ? ASC_STREAM_ZEROCOPY : ASC_STREAM_GATHER;
+    const std::string sm = a.get("--stream", "gather");
+    cfg.stream_mode = sm == "zerocopy" ? ASC_STREAM_ZEROCOPY : sm == "hybrid" ? ASC_STREAM_HYBRID : ASC_STREAM_GATHER;
     cfg.value_convention = a.get("--values", "ref") == "zero" ? ASC_VALUES_ZERO_BASED : ASC_VALUES_REFERENCE;
     cfg.verbose = (int)a.num("--verbose", 0);
     cfg.profile = 1;

@@ -361,7 +361,7 @@ static int plan_dense_chunks(asc_graph* g, TravState* t) {
     t->ch_v.clear();
     t->ch_e.clear();
     const asc_config& cfg = g->ctx->cfg;
-    if (cfg.stream_mode != ASC_STREAM_GATHER || cfg.dense_threshold > 1.0) return ASC_OK;
+    if (cfg.stream_mode == ASC_STREAM_ZEROCOPY || cfg.dense_threshold > 1.0) return ASC_OK;
     if (!g->packed && !g->zc_tail.p) return ASC_OK;
     const uint64_t V = g->h.V, vs = g->static_vertices;
     const uint64_t* off = g->h.off.p;
@@ -431,9 +431,10 @@ int trav_ensure(asc_graph* g) {
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
         for (int i = 0; i < 2; i++) {
-            if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER) {
+            if (g->ctx->cfg.stream_mode != ASC_STREAM_ZEROCOPY) {
                 ASC_HIP(hipMalloc(&t->part[i], EB * t->part_edges));
-                ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
+                if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER)
+                    ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
             }
             ASC_HIP(hipEventCreateWithFlags(&t->ev_copy[i], hipEventDisableTiming));
             ASC_HIP(hipEventCreateWithFlags(&t->ev_kern[i], hipEventDisableTiming));
@@ -663,8 +664,27 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             dense_edges += e1 - e0;
         }
-        // the rest: host gather into pinned staging, double-buffered copies
         const uint64_t Ts = t->h_prefix[n_sparse];
+        if (ctx->cfg.stream_mode == ASC_STREAM_HYBRID) {
+            // the rest read by the kernel from the mapped pinned tail
+            if (Ts > 0) {
+                void* dptr = nullptr;
+                ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
+                p.zc = (const EdgeT*)dptr;
+                p.zc32 = (const uint32_t*)dptr;
+                p.tail_base = S;
+                p.lo = 0;
+                p.hi = Ts;
+                ASC_TRY(launch_part(true));
+                nb++;
+            }
+            rc.parts += nb;
+            rc.streamed_edges += Ts + dense_edges;
+            rc.dense_edges += dense_edges;
+            rc.overload_ms += now_ms() - t_begin;
+            return ASC_OK;
+        }
+        // the rest: host gather into pinned staging, double-buffered copies
         const uint64_t PE = t->part_edges;
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {

@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--seed", type=int, default=24)
     ap.add_argument("--budget-frac", type=float, default=0.4)
     ap.add_argument("--part-mb", type=int, default=64)
-    ap.add_argument("--stream-mode", choices=["gather", "zerocopy"], default="gather")
+    ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
@@ -307,7 +307,7 @@ def main():
         f"({time.perf_counter() - t:.1f}s)")
     s_e, s_o = 8, 8
     budget = int(args.budget_frac * E * s_e)
-    mode = _abi.STREAM_GATHER if args.stream_mode == "gather" else _abi.STREAM_ZEROCOPY
+    mode = {"gather": _abi.STREAM_GATHER, "zerocopy": _abi.STREAM_ZEROCOPY, "hybrid": _abi.STREAM_HYBRID}[args.stream_mode]
     ctx = A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, stream_mode=mode, profile=True,
                     sssp_delta=args.delta)
     g = ctx.from_arrays(off, ew)

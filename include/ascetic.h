@@ -85,8 +85,11 @@ typedef enum {
 typedef enum {
     ASC_STREAM_GATHER = 0,  /* host threads gather active on-demand edges into
                                pinned staging; double-buffered hipMemcpyAsync */
-    ASC_STREAM_ZEROCOPY = 1 /* kernels read on-demand edges from mapped pinned
+    ASC_STREAM_ZEROCOPY = 1,/* kernels read on-demand edges from mapped pinned
                                host memory directly                         */
+    ASC_STREAM_HYBRID = 2   /* dense chunks copied whole by DMA (as in
+                               GATHER); the remaining active edges read by
+                               the kernels from mapped pinned memory        */
 } asc_stream_mode;
 
 /* BFS direction (asc_config.bfs_direction). */

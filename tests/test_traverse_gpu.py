@@ -68,7 +68,7 @@ def test_bfs_sssp_cc_resident(gpu_ctx, name, mk, off32):
             assert np.array_equal(lv, O.bfs_queue(csr, s))
 
 
-@pytest.mark.parametrize("mode", [_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY])
+@pytest.mark.parametrize("mode", [_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY, _abi.STREAM_HYBRID])
 @pytest.mark.parametrize("name,mk", CASES[3:])
 def test_streaming_matches_resident(name, mk, mode):
     say("streaming", name, "mode", mode)
