@@ -195,7 +195,7 @@ class ServerOpts(C.Structure):
         ("lanes", C.c_int),
         ("state_budget_bytes", C.c_uint64),
         ("queue_capacity", C.c_int),
-        ("pad", C.c_int),
+        ("bfs_direction", C.c_int),
     ]
 
 
@@ -207,6 +207,7 @@ class ServerStats(C.Structure):
         ("lanes_bfs", C.c_int),
         ("lanes_sssp", C.c_int),
         ("busy_ms", C.c_double),
+        ("bottom_up_steps", C.c_uint64),
     ]
 
 

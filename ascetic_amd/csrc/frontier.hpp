@@ -17,18 +17,26 @@ __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
 }
 
 // Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
-// and routing of an improved vertex.
+// and routing of an improved vertex, in two steps so that a wave can issue
+// the probes of several edges before any of their atomics: relax_probe loads
+// the destination's word (BFS: visited bits; SSSP/CC: value), relax_commit
+// decides from it and performs the atomics.
 template <int ALGO, class OffT, class EdgeT>
-__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
+__device__ __forceinline__ uint32_t relax_probe(const TravArgs<OffT, EdgeT>& a, uint32_t v) {
+    return ALGO == ALG_BFS ? a.visited[v >> 5] : a.val[v];
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ int relax_commit(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level,
+                                            uint32_t probe) {
     if (ALGO == ALG_BFS) {
         const uint32_t bit = 1u << (v & 31);
-        uint32_t* word = a.visited + (v >> 5);
-        if (*word & bit) return TO_NONE;
-        if (atomicOr(word, bit) & bit) return TO_NONE;
+        if (probe & bit) return TO_NONE;
+        if (atomicOr(a.visited + (v >> 5), bit) & bit) return TO_NONE;
         a.val[v] = cand;
         return v < a.vs ? TO_RES : TO_PEND;
     } else {
-        if (cand >= a.val[v]) return TO_NONE;
+        if (cand >= probe) return TO_NONE;
         const uint32_t old = atomicMin(a.val + v, cand);
         if (old <= cand) return TO_NONE;
         if (ALGO == ALG_SSSP && cand >= a.theta)
@@ -37,6 +45,15 @@ __device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v,
         return atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
     }
 }
+
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
+    return relax_commit<ALGO>(a, v, cand, level, relax_probe<ALGO>(a, v));
+}
+
+// Edges each lane takes per pass of an edge loop: the loads of kUnroll edges,
+// then their probes, are in flight together.
+constexpr int kUnroll = 4;
 
 template <int ALGO>
 struct Pushes {
@@ -111,20 +128,33 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
         s_sv[tid] = sv;
         __syncthreads();
         const uint32_t cnt = min(n - base, (uint32_t)kBlock);
-        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
-            const uint32_t e = eb + lane;
-            int to = TO_NONE;
-            uint32_t v = 0;
-            if (e < total) {
+        for (uint32_t eb = w * kWave * kUnroll; eb < total; eb += kBlock * kUnroll) {
+            uint32_t kk[kUnroll], vv[kUnroll], cc[kUnroll], pr[kUnroll];
+            EdgeT ed[kUnroll];
+            // lanes past the end load the last edge again (no branch around
+            // the loads, so all of them are in flight together)
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const uint32_t e = min(eb + j * kWave + lane, total - 1);
                 uint32_t k = 0;
 #pragma unroll
                 for (uint32_t step = kBlock / 2; step > 0; step >>= 1)
                     if (k + step < cnt && s_pref[k + step] <= e) k += step;
-                const EdgeT ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
-                v = edge_dst(ed);
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], edge_w(ed)), level);
+                ed[j] = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
+                kk[j] = k;
             }
-            pq.push(a, nxt, to, v);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                vv[j] = edge_dst(ed[j]);
+                cc[j] = candidate<ALGO>(s_sv[kk[j]], edge_w(ed[j]));
+                pr[j] = relax_probe<ALGO>(a, vv[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                int to = TO_NONE;
+                if (eb + j * kWave + lane < total) to = relax_commit<ALGO>(a, vv[j], cc[j], level, pr[j]);
+                pq.push(a, nxt, to, vv[j]);
+            }
         }
         my_edges += total;
         my_verts += cnt;
@@ -174,21 +204,31 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
-        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
-            const uint32_t rr = r + lane;
-            int to = TO_NONE;
-            uint32_t v = 0;
-            if (rr < tlen) {
-                const uint64_t e = t0 + rr;
+        for (uint32_t r = w * kWave * kUnroll; r < tlen; r += kBlock * kUnroll) {
+            uint32_t kk[kUnroll], vv[kUnroll], cc[kUnroll], pr[kUnroll];
+            EdgeT ed[kUnroll];
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const uint64_t e = t0 + min(r + j * kWave + lane, tlen - 1);
                 uint32_t k = 0;
 #pragma unroll
-                for (uint32_t j = 1; j < (uint32_t)kWin; j++)
-                    if (j < nwin && s_hb[j] <= e) k = j;
-                const EdgeT ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
-                v = edge_dst(ed);
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_hsv[k], edge_w(ed)), level);
+                for (uint32_t i = 1; i < (uint32_t)kWin; i++)
+                    if (i < nwin && s_hb[i] <= e) k = i;
+                ed[j] = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
+                kk[j] = k;
             }
-            pq.push(a, nxt, to, v);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                vv[j] = edge_dst(ed[j]);
+                cc[j] = candidate<ALGO>(s_hsv[kk[j]], edge_w(ed[j]));
+                pr[j] = relax_probe<ALGO>(a, vv[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                int to = TO_NONE;
+                if (r + j * kWave + lane < tlen) to = relax_commit<ALGO>(a, vv[j], cc[j], level, pr[j]);
+                pq.push(a, nxt, to, vv[j]);
+            }
         }
         my_edges += tlen;
         __syncthreads();

@@ -167,7 +167,7 @@ void asc_server_opts_init(asc_server_opts* o) {
     o->lanes = 0;  // the graph's tuned batch width (64 until tuned)
     o->state_budget_bytes = 0;
     o->queue_capacity = 4096;
-    o->pad = 0;
+    o->bfs_direction = ASC_BFS_AUTO;
 }
 
 }  // extern "C"

@@ -12,6 +12,13 @@
 //         with wave lane q relaxing query q, so a relaxation reads and
 //         updates 4*lanes contiguous bytes; improved lanes are ORed into the
 //         destination's 64-bit frontier word.
+//   On a symmetric graph a BFS lane whose next frontier holds more than
+//   1/14 of its unexplored edges switches to bottom-up steps (Beamer et al.),
+//   and back once its frontier shrinks below V/24 vertices: every vertex that
+//   some bottom-up lane has not seen scans its own adjacency for frontier
+//   bits of those lanes, X[u] & need, and stops once every needed lane is
+//   found.  Bottom-up runs before the top-down kernels of the same step,
+//   which skip the bits of bottom-up lanes.
 // One engine step advances every admitted query by one frontier round, using
 // the same edge-balanced workgroup expansion and grid-cooperative heavy list
 // as the single-query engine.  A lane whose query produced no new frontier bit
@@ -56,6 +63,7 @@ struct BatchArgs {
     unsigned long long* heavy_bits;
     unsigned long long* heavy_base;
     const uint32_t* s0;              // step at which each lane was seeded
+    unsigned long long td_mask;      // BFS lanes expanded top-down this step
     uint64_t V;
     uint32_t stride;                 // SSSP row length (lanes)
     uint32_t heavy_thr;
@@ -105,18 +113,20 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a
         unsigned long long bits = 0;
         if (i < n) {
             const uint32_t u = a.q_in[i];
-            bits = a.X[u];
+            bits = a.X[u] & a.td_mask;   // bottom-up lanes' bits were consumed already
             a.X[u] = 0;
-            beg = a.off[u];
-            const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
-            if (d >= a.heavy_thr) {
-                const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
-                const uint32_t idx = (uint32_t)(p >> kHeavyShift);
-                a.heavy_u[idx] = u;
-                a.heavy_bits[idx] = bits;
-                a.heavy_base[idx] = p & kHeavyMask;
-            } else {
-                deg = (uint32_t)d;
+            if (bits) {
+                beg = a.off[u];
+                const uint64_t d = (uint64_t)(a.off[u + 1] - beg);
+                if (d >= a.heavy_thr) {
+                    const unsigned long long p = atomicAdd(&cur->heavy, (1ull << kHeavyShift) | d);
+                    const uint32_t idx = (uint32_t)(p >> kHeavyShift);
+                    a.heavy_u[idx] = u;
+                    a.heavy_bits[idx] = bits;
+                    a.heavy_base[idx] = p & kHeavyMask;
+                } else {
+                    deg = (uint32_t)d;
+                }
             }
         }
         uint32_t total;
@@ -204,6 +214,99 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
     if (lane == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+// One bottom-up step for the lanes in bu_mask (symmetric graph: in-neighbours
+// are the adjacency).  Thread per vertex; four neighbour ids are loaded
+// before their frontier words are probed.
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, EdgeT> a, uint32_t step,
+                                                            unsigned long long bu_mask) {
+    __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    Slot* nxt = a.slots + ((step + 1) & 3);
+    WaveStage<kStageCap> st{s_stage[wave_id()], 0};
+    unsigned long long act = 0;
+    for (uint64_t base = (uint64_t)blockIdx.x * kBlock; base < a.V; base += (uint64_t)gridDim.x * kBlock) {
+        const uint64_t v = base + threadIdx.x;
+        bool push = false;
+        if (v < a.V) {
+            const unsigned long long need = bu_mask & ~a.seen[v];
+            if (need) {
+                const OffT b = a.off[v], e = a.off[v + 1];
+                unsigned long long nw = 0;
+                for (OffT i = b; i < e && nw != need; i += 4) {
+                    uint32_t u[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) u[j] = edge_dst(a.edges[min(i + (OffT)j, e - 1)]);
+#pragma unroll
+                    for (int j = 0; j < 4; j++) nw |= a.X[u[j]] & need;
+                }
+                if (nw) {
+                    const unsigned long long old = atomicOr(a.seen + v, nw);
+                    nw &= ~old;
+                    if (nw) {
+                        act |= nw;
+                        for (unsigned long long m = nw; m; m &= m - 1) {
+                            const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+                            a.lev[(size_t)q * a.V + v] = step - a.s0[q] + 2;
+                        }
+                        push = atomicOr(a.Y + v, nw) == 0;
+                    }
+                }
+            }
+        }
+        st.push(push, (uint32_t)v, a.q_out, &nxt->n_res);
+    }
+    st.flush(a.q_out, &nxt->n_res);
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) act |= __shfl_xor(act, d, kWave);
+    if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
+}
+
+// Per lane q: vertices with bit q in Y (cnt[q]) and their summed degrees
+// (cnt[64 + q]).  A wave takes 64 consecutive vertices; for each bit present
+// in the wave, a ballot counts the vertices and a wave sum adds the degrees.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const unsigned long long* Y, uint64_t V,
+                                                        unsigned long long* cnt) {
+    __shared__ unsigned long long s_n[kMaxLanes], s_m[kMaxLanes];
+    const uint32_t lane = lane_id();
+    if (threadIdx.x < kMaxLanes) {
+        s_n[threadIdx.x] = 0;
+        s_m[threadIdx.x] = 0;
+    }
+    __syncthreads();
+    unsigned long long my_n = 0, my_m = 0;
+    const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
+    for (uint64_t wv = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); wv * kWave < V; wv += nw) {
+        const uint64_t v = wv * kWave + lane;
+        const unsigned long long y = v < V ? Y[v] : 0ull;
+        const unsigned long long any = wave_ballot(y != 0);
+        if (!any) continue;
+        const unsigned long long d = y ? (unsigned long long)(off[v + 1] - off[v]) : 0ull;
+        unsigned long long present = y;
+#pragma unroll
+        for (int k = kWave / 2; k > 0; k >>= 1) present |= __shfl_xor(present, k, kWave);
+        for (unsigned long long m = present; m; m &= m - 1) {
+            const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+            const bool b = (y >> q) & 1ull;
+            const unsigned long long n_q = __popcll(wave_ballot(b));
+            const unsigned long long m_q = wave_sum(b ? d : 0ull);
+            if (lane == q) {
+                my_n += n_q;
+                my_m += m_q;
+            }
+        }
+    }
+    if (my_n) {
+        atomicAdd(&s_n[lane], my_n);
+        atomicAdd(&s_m[lane], my_m);
+    }
+    __syncthreads();
+    if (threadIdx.x < kMaxLanes && s_n[threadIdx.x]) {
+        atomicAdd(cnt + threadIdx.x, s_n[threadIdx.x]);
+        atomicAdd(cnt + kMaxLanes + threadIdx.x, s_m[threadIdx.x]);
+    }
 }
 
 // -- SSSP lanes -----------------------------------------------------------------
@@ -455,6 +558,14 @@ struct Engine {
     unsigned long long busy = 0;
     uint32_t step = 0;
     bool launched = false;
+    // BFS direction per lane: bottom-up lanes, next frontier vertices and
+    // edges (from k_lane_counts), unexplored edges
+    bool bu_ok = false;
+    bool launched_bu = false;         // the last step ran a bottom-up kernel
+    unsigned long long bu = 0;
+    uint64_t nf[kMaxLanes] = {}, prev_nf[kMaxLanes] = {}, mf[kMaxLanes] = {}, mu[kMaxLanes] = {};
+    unsigned long long* d_cnt = nullptr;
+    HostBuf<unsigned long long> h_cnt;
 };
 
 }  // namespace asc
@@ -468,7 +579,7 @@ struct asc_server {
     std::condition_variable cv_work, cv_done;
     std::deque<asc::SQuery*> pending[2];
     std::unordered_map<uint64_t, asc::SQuery*> tickets;
-    uint64_t next_ticket = 1, submitted = 0, completed = 0, steps = 0, outstanding = 0;
+    uint64_t next_ticket = 1, submitted = 0, completed = 0, steps = 0, outstanding = 0, bu_steps = 0;
     double busy_ms = 0;
     bool stop = false;
     int error = ASC_OK;
@@ -481,7 +592,7 @@ static void engine_free(Engine& e) {
         if (p) (void)hipFree(p);
     };
     f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
-    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging);
+    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt);
     if (e.stream) (void)hipStreamDestroy(e.stream);
     e.stream = nullptr;
 }
@@ -516,6 +627,10 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     ASC_HIP(hipMalloc(&e.heavy_base, 8 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&e.s0, 4 * kMaxLanes));
     ASC_TRY(e.h_slot.alloc(1, true));
+    if (kind == ASC_QUERY_BFS) {
+        ASC_HIP(hipMalloc(&e.d_cnt, 16 * kMaxLanes));
+        ASC_TRY(e.h_cnt.alloc(2 * kMaxLanes, true));
+    }
     ASC_HIP(hipStreamSynchronize(e.stream));
     return ASC_OK;
 }
@@ -537,10 +652,42 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
     a.heavy_bits = e.heavy_bits;
     a.heavy_base = e.heavy_base;
     a.s0 = e.s0;
+    a.td_mask = ~e.bu;
     a.V = e.V;
     a.stride = e.stride;
     a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
     return a;
+}
+
+constexpr uint64_t kMsAlpha = 14;  // lane goes bottom-up when its frontier edges > unexplored / kMsAlpha
+constexpr uint64_t kMsBeta = 24;   // and back when its frontier shrinks below V / kMsBeta vertices
+
+// Chooses the direction of every BFS lane for the coming step from the
+// counts of its frontier (fresh lanes start top-down).
+static void choose_directions(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
+    unsigned long long fresh = 0;
+    for (SQuery* q : admitted) {
+        const uint32_t l = q->lane;
+        fresh |= 1ull << l;
+        const uint64_t d = g->h.off[q->src + 1] - g->h.off[q->src];
+        e.nf[l] = 1;
+        e.prev_nf[l] = 0;
+        e.mf[l] = d;
+        e.mu[l] = g->h.E - d;
+    }
+    e.bu &= ~fresh;
+    if (e.bu_ok) {
+        for (int l = 0; l < e.lanes; l++) {
+            const unsigned long long bit = 1ull << l;
+            if (!(e.busy & bit) || (fresh & bit)) continue;
+            if (!(e.bu & bit)) {
+                if (e.mf[l] > e.mu[l] / kMsAlpha) e.bu |= bit;
+            } else if (e.nf[l] < e.V / kMsBeta && e.nf[l] < e.prev_nf[l]) {
+                e.bu &= ~bit;
+            }
+        }
+    }
+    e.bu &= e.busy;
 }
 
 // Resets and seeds newly admitted lanes, then launches one step.
@@ -548,6 +695,7 @@ template <class OffT, class EdgeT>
 static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
     const int grid = g->ctx->num_cus * (g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
     const uint32_t unreached = (uint32_t)(e.V + 1);
+    if (e.kind == ASC_QUERY_BFS) choose_directions(g, e, admitted);
     auto a = engine_args<OffT, EdgeT>(g, e);
     if (!admitted.empty()) {
         SeedArgs sa{};
@@ -578,8 +726,17 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
         }
     }
     if (e.kind == ASC_QUERY_BFS) {
+        e.launched_bu = e.bu != 0;
+        if (e.bu)
+            hipLaunchKernelGGL((k_msbfs_bottom_up<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step,
+                               e.bu);
         hipLaunchKernelGGL((k_msbfs_local<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
         hipLaunchKernelGGL((k_msbfs_heavy<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+        if (e.bu_ok) {
+            ASC_HIP(hipMemsetAsync(e.d_cnt, 0, 16 * kMaxLanes, e.stream));
+            hipLaunchKernelGGL((k_lane_counts<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a.off, a.Y, e.V, e.d_cnt);
+            ASC_HIP(hipMemcpyAsync(e.h_cnt.p, e.d_cnt, 16 * kMaxLanes, hipMemcpyDeviceToHost, e.stream));
+        }
     }
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipMemcpyAsync(e.h_slot.p, e.slots + ((e.step + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, e.stream));
@@ -593,6 +750,14 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
     ASC_HIP(hipStreamSynchronize(e.stream));
     e.launched = false;
     const unsigned long long act = e.h_slot[0].pad[0];
+    if (e.kind == ASC_QUERY_BFS && e.bu_ok) {
+        for (int q = 0; q < e.lanes; q++) {
+            e.prev_nf[q] = e.nf[q];
+            e.nf[q] = e.h_cnt[q];
+            e.mf[q] = e.h_cnt[kMaxLanes + q];
+            e.mu[q] = e.mu[q] > e.mf[q] ? e.mu[q] - e.mf[q] : 0;
+        }
+    }
     for (int q = 0; q < e.lanes; q++) {
         SQuery* qq = e.lane_q[q];
         if (!qq) continue;
@@ -665,6 +830,7 @@ static void server_loop(asc_server* s) {
             std::lock_guard<std::mutex> lk(s->mu);
             s->busy_ms += now_ms() - t_step;
             s->steps++;
+            if (s->eng[ASC_QUERY_BFS].launched_bu) s->bu_steps++;
             if (r != ASC_OK) {
                 // a device error fails every query in flight or waiting
                 s->error = r;
@@ -726,6 +892,7 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
     s->g = g;
     s->opts = o;
     int r = engine_init(s->eng[ASC_QUERY_BFS], ASC_QUERY_BFS, lanes, V);
+    s->eng[ASC_QUERY_BFS].bu_ok = o.bfs_direction == ASC_BFS_AUTO && resident_symmetric(g);
     if (r == ASC_OK) r = engine_init(s->eng[ASC_QUERY_SSSP], ASC_QUERY_SSSP, weighted ? lanes : 0, V);
     if (r != ASC_OK) {
         engine_free(s->eng[0]);
@@ -825,6 +992,7 @@ int asc_server_get_stats(asc_server* s, asc_server_stats* st) {
     st->lanes_bfs = s->eng[ASC_QUERY_BFS].lanes;
     st->lanes_sssp = s->eng[ASC_QUERY_SSSP].lanes;
     st->busy_ms = s->busy_ms;
+    st->bottom_up_steps = s->bu_steps;
     return ASC_OK;
 }
 

@@ -83,31 +83,39 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)(t1 - t0);
-        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
-            const uint32_t rr = r + lane;
-            int to = TO_NONE;
-            uint32_t v = 0;
-            if (rr < tlen) {
-                const uint64_t e = t0 + rr;
+        for (uint32_t r = w * kWave * kUnroll; r < tlen; r += kBlock * kUnroll) {
+            uint32_t kk[kUnroll], vv[kUnroll], ww[kUnroll], pr[kUnroll];
+            // lanes past the end load the last edge again (no branch around
+            // the loads, so all of them are in flight together)
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                const uint64_t e = t0 + min(r + j * kWave + lane, tlen - 1);
                 uint32_t k = 0;
 #pragma unroll
                 for (uint32_t step = kTile / 2; step > 0; step >>= 1)
                     if (k + step < m && s_p[k + step] <= e) k += step;
-                uint32_t w;
                 if (PK) {
                     const uint32_t x = ZC ? p.zc32[s_src[k] + (e - s_p[k])] : p.part32[e - p.lo];
-                    v = x & p.pk_mask;
-                    w = (x >> p.pk_shift) + p.pk_wmin;
+                    vv[j] = x & p.pk_mask;
+                    ww[j] = (x >> p.pk_shift) + p.pk_wmin;
                 } else {
                     EdgeT ed;
                     if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
                     else ed = p.part[e - p.lo];
-                    v = edge_dst(ed);
-                    w = edge_w(ed);
+                    vv[j] = edge_dst(ed);
+                    ww[j] = edge_w(ed);
                 }
-                to = relax<ALGO>(a, v, candidate<ALGO>(s_sv[k], w), level);
+                kk[j] = k;
             }
-            pq.push(a, nxt, to, v);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) pr[j] = relax_probe<ALGO>(a, vv[j]);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) {
+                int to = TO_NONE;
+                if (r + j * kWave + lane < tlen)
+                    to = relax_commit<ALGO>(a, vv[j], candidate<ALGO>(s_sv[kk[j]], ww[j]), level, pr[j]);
+                pq.push(a, nxt, to, vv[j]);
+            }
         }
         my_edges += tlen;
         my_verts += m;

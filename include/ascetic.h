@@ -366,7 +366,9 @@ typedef struct {
     uint64_t state_budget_bytes; /* device bytes for per-query state (4 V bytes
                                     per lane); 0 = half the free memory       */
     int queue_capacity;          /* max submitted-but-unfinished queries      */
-    int pad;
+    int bfs_direction;           /* asc_bfs_direction: AUTO lets a BFS lane
+                                    whose frontier is large take bottom-up
+                                    steps on a symmetric graph               */
 } asc_server_opts;
 void asc_server_opts_init(asc_server_opts* o);
 
@@ -384,6 +386,7 @@ typedef struct {
     uint64_t steps;              /* engine steps run                          */
     int lanes_bfs, lanes_sssp;   /* lanes granted by the memory budget        */
     double busy_ms;              /* wall time with at least one query in flight */
+    uint64_t bottom_up_steps;    /* BFS engine steps with bottom-up lanes      */
 } asc_server_stats;
 
 /* Starts a background thread that admits and advances queries. */

@@ -57,6 +57,38 @@ def test_mix_equals_alone(wgraph):
         assert np.array_equal(o, ref), (k, s)
 
 
+def test_bottom_up_lanes_exact(wgraph):
+    """BFS lanes that switch to bottom-up steps (symmetric graph) give the
+    same levels as top-down-only serving, single runs and the oracle."""
+    g, off, ew = wgraph
+    V = len(off) - 1
+    deg = np.diff(off)
+    rng = np.random.default_rng(7)
+    cand = np.nonzero(deg > 0)[0]
+    srcs = [int(x) for x in rng.choice(cand, 100, replace=False)] + [0, int(np.argmax(deg))]
+    csr = O.CSR(off, ew[:, 0], ew[:, 1])
+    res = {}
+    for direction in (_abi.BFS_AUTO, _abi.BFS_TOP_DOWN):
+        outs = [np.empty(V, np.uint32) for _ in srcs]
+        say("bfs serving, direction", direction)
+        with A.Server(g, lanes=64, bfs_direction=direction) as srv:
+            tickets = [srv.submit(A.Server.BFS, s, o) for s, o in zip(srcs, outs)]
+            for t in tickets:
+                srv.wait(t)
+            st = srv.stats()
+        if direction == _abi.BFS_AUTO:
+            assert st["bottom_up_steps"] > 0
+        else:
+            assert st["bottom_up_steps"] == 0
+        res[direction] = outs
+    for i, s in enumerate(srcs):
+        assert np.array_equal(res[_abi.BFS_AUTO][i], res[_abi.BFS_TOP_DOWN][i]), s
+        alone, _ = g.bfs(s)
+        assert np.array_equal(res[_abi.BFS_AUTO][i], alone), s
+    for i, s in enumerate(srcs[:10]):
+        assert np.array_equal(res[_abi.BFS_AUTO][i], O.bfs_queue(csr, s)), s
+
+
 def test_staggered_arrivals_from_threads(wgraph):
     g, off, ew = wgraph
     V = len(off) - 1
