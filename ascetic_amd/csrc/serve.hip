@@ -263,11 +263,14 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, Edge
     if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
-// Per lane q: vertices with bit q in Y (cnt[q]) and their summed degrees
-// (cnt[64 + q]).  A wave takes 64 consecutive vertices; for each bit present
-// in the wave, a ballot counts the vertices and a wave sum adds the degrees.
+// Per lane q: vertices with bit q in Y (cnt[q]) and, for the lanes in
+// `edge_mask`, their summed degrees (cnt[64 + q]), over the queue q of the
+// vertices whose Y word is non-zero (count in slot->n_res).  A wave takes 64
+// queue entries; for each bit present in the wave a ballot counts the
+// vertices, and for the masked bits a wave sum adds the degrees.
 template <class OffT>
-__global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const unsigned long long* Y, uint64_t V,
+__global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const unsigned long long* Y, const uint32_t* q,
+                                                        const Slot* slot, unsigned long long edge_mask,
                                                         unsigned long long* cnt) {
     __shared__ unsigned long long s_n[kMaxLanes], s_m[kMaxLanes];
     const uint32_t lane = lane_id();
@@ -276,36 +279,39 @@ __global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const u
         s_m[threadIdx.x] = 0;
     }
     __syncthreads();
+    const uint32_t n = slot->n_res;
     unsigned long long my_n = 0, my_m = 0;
     const uint64_t nw = (uint64_t)gridDim.x * kWavesPerBlock;
-    for (uint64_t wv = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); wv * kWave < V; wv += nw) {
-        const uint64_t v = wv * kWave + lane;
-        const unsigned long long y = v < V ? Y[v] : 0ull;
-        const unsigned long long any = wave_ballot(y != 0);
-        if (!any) continue;
-        const unsigned long long d = y ? (unsigned long long)(off[v + 1] - off[v]) : 0ull;
+    for (uint64_t wv = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); wv * kWave < n; wv += nw) {
+        const uint64_t i = wv * kWave + lane;
+        unsigned long long y = 0, d = 0;
+        if (i < n) {
+            const uint32_t v = q[i];
+            y = Y[v];
+            if (y & edge_mask) d = (unsigned long long)(off[v + 1] - off[v]);
+        }
         unsigned long long present = y;
 #pragma unroll
         for (int k = kWave / 2; k > 0; k >>= 1) present |= __shfl_xor(present, k, kWave);
         for (unsigned long long m = present; m; m &= m - 1) {
-            const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
-            const bool b = (y >> q) & 1ull;
-            const unsigned long long n_q = __popcll(wave_ballot(b));
-            const unsigned long long m_q = wave_sum(b ? d : 0ull);
-            if (lane == q) {
-                my_n += n_q;
-                my_m += m_q;
+            const uint32_t b = (uint32_t)__ffsll((long long)m) - 1;
+            const bool on = (y >> b) & 1ull;
+            const unsigned long long n_b = __popcll(wave_ballot(on));
+            if (lane == b) my_n += n_b;
+            if ((edge_mask >> b) & 1ull) {
+                const unsigned long long m_b = wave_sum(on ? d : 0ull);
+                if (lane == b) my_m += m_b;
             }
         }
     }
     if (my_n) {
         atomicAdd(&s_n[lane], my_n);
-        atomicAdd(&s_m[lane], my_m);
+        if (my_m) atomicAdd(&s_m[lane], my_m);
     }
     __syncthreads();
     if (threadIdx.x < kMaxLanes && s_n[threadIdx.x]) {
         atomicAdd(cnt + threadIdx.x, s_n[threadIdx.x]);
-        atomicAdd(cnt + kMaxLanes + threadIdx.x, s_m[threadIdx.x]);
+        if (s_m[threadIdx.x]) atomicAdd(cnt + kMaxLanes + threadIdx.x, s_m[threadIdx.x]);
     }
 }
 
@@ -534,6 +540,7 @@ struct SQuery {
     uint32_t* out = nullptr;
     double t_sub = 0, t_adm = 0, t_done = 0;
     uint32_t lane = 0, steps = 0;
+    uint64_t sub_step = 0;            // server step count at submission
     bool done = false;
 };
 
@@ -734,7 +741,8 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
         hipLaunchKernelGGL((k_msbfs_heavy<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
         if (e.bu_ok) {
             ASC_HIP(hipMemsetAsync(e.d_cnt, 0, 16 * kMaxLanes, e.stream));
-            hipLaunchKernelGGL((k_lane_counts<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a.off, a.Y, e.V, e.d_cnt);
+            hipLaunchKernelGGL((k_lane_counts<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a.off, a.Y, a.q_out,
+                               e.slots + ((e.step + 1) & 3), ~e.bu, e.d_cnt);
             ASC_HIP(hipMemcpyAsync(e.h_cnt.p, e.d_cnt, 16 * kMaxLanes, hipMemcpyDeviceToHost, e.stream));
         }
     }
@@ -784,6 +792,20 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
     return ASC_OK;
 }
 
+constexpr uint64_t kMaxHoldSteps = 3;  // longest a waiting BFS query is held back for a group
+
+// BFS lanes that take bottom-up steps together finish their scans early when
+// they are at the same level, so while the engine is busy, freed BFS lanes
+// are refilled as a group of at least a quarter of the lanes, or once the
+// oldest waiting query has waited kMaxHoldSteps steps.  Otherwise waiting
+// queries are admitted at the first step boundary with a free lane.
+static bool admit_now(const asc_server* s, const Engine& e) {
+    if (e.kind != ASC_QUERY_BFS || !e.bu_ok || !e.busy || s->pending[e.kind].empty()) return true;
+    const int free_lanes = e.lanes - __builtin_popcountll(e.busy);
+    if (free_lanes >= std::max(1, e.lanes / 4)) return true;
+    return s->steps - s->pending[e.kind].front()->sub_step >= kMaxHoldSteps;
+}
+
 template <class OffT, class EdgeT>
 static void server_loop(asc_server* s) {
     (void)hipSetDevice(s->g->ctx->device);
@@ -801,6 +823,7 @@ static void server_loop(asc_server* s) {
             for (int k = 0; k < 2; k++) {
                 Engine& e = s->eng[k];
                 admitted[k].clear();
+                if (!admit_now(s, e)) continue;
                 for (int q = 0; q < e.lanes && !s->pending[k].empty(); q++) {
                     if (e.busy & (1ull << q)) continue;
                     SQuery* qq = s->pending[k].front();
@@ -947,6 +970,7 @@ int asc_submit(asc_server* s, int kind, uint32_t source, uint32_t* out, uint64_t
             return ASC_ERR_BUSY;
         }
         q->ticket = s->next_ticket++;
+        q->sub_step = s->steps;
         s->tickets[q->ticket] = q;
         s->pending[kind].push_back(q);
         s->submitted++;
