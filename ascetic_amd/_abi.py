@@ -208,6 +208,7 @@ class ServerStats(C.Structure):
         ("lanes_sssp", C.c_int),
         ("busy_ms", C.c_double),
         ("bottom_up_steps", C.c_uint64),
+        ("fallbacks", C.c_uint64),
     ]
 
 

@@ -8,10 +8,13 @@
 //         (u, v) advances every lane whose frontier holds u:
 //         new = X[u] & ~seen[v].  Levels are kept lane-major (lev[q][v]), so
 //         a finished query's result is one contiguous copy.
-//   SSSP  distance lanes: dist[v][q] vertex-major, one wavefront per edge
-//         with wave lane q relaxing query q, so a relaxation reads and
-//         updates 4*lanes contiguous bytes; improved lanes are ORed into the
-//         destination's 64-bit frontier word.
+//   SSSP  distance lanes: dist[v][q] vertex-major in 16 bits (a row of 64
+//         lanes is one 128-byte line), one thread per edge relaxing every
+//         lane whose frontier holds the edge's source; improved lanes are
+//         ORed into the destination's 64-bit frontier word.  A candidate
+//         above 65534 marks the lane as overflowed instead; such a query is
+//         answered by the single-query engine when it leaves its lane, so
+//         every result is exact on any graph.
 //   On a symmetric graph a BFS lane whose next frontier holds more than
 //   1/14 of its unexplored edges switches to bottom-up steps (Beamer et al.),
 //   and back once its frontier shrinks below V/24 vertices: every vertex that
@@ -55,7 +58,8 @@ struct BatchArgs {
     unsigned long long* Y;           // frontier bits of the next step
     unsigned long long* seen;        // BFS
     uint32_t* lev;                   // BFS: lev[q * V + v]
-    uint32_t* dist;                  // SSSP: dist[v * stride + q]
+    uint16_t* dist;                  // SSSP: dist[v * stride + q], 0xFFFF = unreached
+    unsigned long long* ovf;         // SSSP: lanes whose distances left 16 bits
     const uint32_t* q_in;
     uint32_t* q_out;
     Slot* slots;                     // pad[0] of slot (s+1)%4: lanes that advanced in step s
@@ -317,39 +321,61 @@ __global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const u
 
 // -- SSSP lanes -----------------------------------------------------------------
 
+constexpr uint32_t kD16Inf = 0xFFFFu;   // unreached in a 16-bit distance lane
+
 // Relaxes edge (u, v, w) for every lane in `bits` (one thread per edge; the
-// lanes of one vertex share its distance row, so the loop stays within one
-// or a few cache lines of dist[v][*]).  Returns true when v must be queued.
-// Rows are `stride` (a multiple of 4) u32 long and 16-byte aligned; phase 1
-// reads both rows in 16-byte groups with no atomics in between, so the loads
-// of all active groups are in flight together; phase 2 issues atomics only
-// for lanes whose candidate beats the value read.
+// lanes of one vertex share its distance row).  Rows are `stride` (a multiple
+// of 8) u16 long and 16-byte aligned; phase 1 reads both rows in 16-byte
+// groups of 8 lanes with no atomics in between, so the loads of all active
+// groups are in flight together; phase 2 lowers, by compare-and-swap on the
+// 32-bit word holding the lane, only lanes whose candidate beats the value
+// read.  Returns true when v must be queued.
 template <class OffT>
 __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
                                            unsigned long long bits, unsigned long long& act) {
-    const uint4* du = (const uint4*)(a.dist + (size_t)u * a.stride);
-    uint32_t* dv = a.dist + (size_t)v * a.stride;
-    const uint4* dv4 = (const uint4*)dv;
-    unsigned long long maybe = 0;
+    const uint4* du4 = (const uint4*)(a.dist + (size_t)u * a.stride);
+    const uint4* dv4 = (const uint4*)(a.dist + (size_t)v * a.stride);
+    unsigned long long maybe = 0, over = 0;
     for (unsigned long long gm = bits; gm;) {
-        const uint32_t g = ((uint32_t)__ffsll((long long)gm) - 1) >> 2;
-        gm &= ~(0xFull << (4 * g));
-        const uint4 x = du[g], y = dv4[g];
-        const uint32_t nib = (uint32_t)(bits >> (4 * g)) & 0xF;
-        uint32_t hit = 0;
-        hit |= (nib & 1) && x.x + wgt < y.x ? 1u : 0u;
-        hit |= (nib & 2) && x.y + wgt < y.y ? 2u : 0u;
-        hit |= (nib & 4) && x.z + wgt < y.z ? 4u : 0u;
-        hit |= (nib & 8) && x.w + wgt < y.w ? 8u : 0u;
-        maybe |= (unsigned long long)hit << (4 * g);
+        const uint32_t g = ((uint32_t)__ffsll((long long)gm) - 1) >> 3;
+        gm &= ~(0xFFull << (8 * g));
+        const uint4 x = du4[g], y = dv4[g];
+        const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
+        const uint32_t byte = (uint32_t)(bits >> (8 * g)) & 0xFF;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            if (!((byte >> j) & 1)) continue;
+            const uint32_t du = (xs[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+            const uint32_t dv = (ys[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+            const uint32_t c = wgt >= kD16Inf ? kD16Inf : du + wgt;   // no 32-bit wrap
+            if (c >= kD16Inf) {
+                over |= 1ull << (8 * g + j);
+            } else if (c < dv) {
+                maybe |= 1ull << (8 * g + j);
+            }
+        }
     }
+    if (over) atomicOr(a.ovf, over);
     if (!maybe) return false;
-    const uint32_t* dus = (const uint32_t*)du;
+    const uint16_t* dus = a.dist + (size_t)u * a.stride;
+    uint32_t* row = (uint32_t*)(a.dist + (size_t)v * a.stride);
     unsigned long long imp = 0;
     for (unsigned long long m = maybe; m; m &= m - 1) {
         const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
-        const uint32_t cand = dus[q] + wgt;
-        if (atomicMin(dv + q, cand) > cand) imp |= 1ull << q;
+        const uint32_t c = (uint32_t)dus[q] + wgt;
+        uint32_t* word = row + (q >> 1);
+        const uint32_t sh = 16 * (q & 1);
+        uint32_t old = *word;
+        for (;;) {
+            if (((old >> sh) & 0xFFFF) <= c) break;
+            const uint32_t nw = (old & ~(0xFFFFu << sh)) | (c << sh);
+            const uint32_t prev = atomicCAS(word, old, nw);
+            if (prev == old) {
+                imp |= 1ull << q;
+                break;
+            }
+            old = prev;
+        }
     }
     if (!imp) return false;
     act |= imp;
@@ -497,9 +523,9 @@ __global__ void k_clear_seen_bits(unsigned long long* seen, uint64_t V, unsigned
         seen[v] &= ~mask;
 }
 
-__global__ void k_reset_dist(uint32_t* dist, uint64_t V, uint32_t stride, unsigned long long mask, uint32_t inf) {
+__global__ void k_reset_dist(uint16_t* dist, uint64_t V, uint32_t stride, unsigned long long mask) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
-        for (unsigned long long m = mask; m; m &= m - 1) dist[v * stride + (__ffsll((long long)m) - 1)] = inf;
+        for (unsigned long long m = mask; m; m &= m - 1) dist[v * stride + (__ffsll((long long)m) - 1)] = kD16Inf;
 }
 
 template <class OffT, class EdgeT>
@@ -514,6 +540,7 @@ __global__ void k_seed(BatchArgs<OffT, EdgeT> a, SeedArgs sa, uint32_t step, uin
         a.lev[(size_t)q * a.V + src] = 1;
     } else {
         a.dist[(size_t)src * a.stride + q] = 1;
+        atomicAnd(a.ovf, ~bit);
     }
     if (atomicOr(a.X + src, bit) == 0) {
         const uint32_t pos = atomicAdd(&a.slots[step & 3].n_res, 1u);
@@ -526,9 +553,12 @@ __global__ void k_lane_fill(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-__global__ void k_extract_lane(const uint32_t* dist, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
-        out[v] = dist[v * stride + lane];
+// Result column of one lane, unreached as V + 1.
+__global__ void k_extract_lane(const uint16_t* dist, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t d = dist[v * stride + lane];
+        out[v] = d == kD16Inf ? (uint32_t)(V + 1) : d;
+    }
 }
 
 // -- host -----------------------------------------------------------------------
@@ -552,7 +582,9 @@ struct Engine {
     hipStream_t stream = nullptr;
     unsigned long long* bits[2] = {nullptr, nullptr};
     unsigned long long* seen = nullptr;
-    uint32_t* lanes_state = nullptr;  // lev (BFS) or dist (SSSP)
+    uint32_t* lanes_state = nullptr;  // lev (BFS, u32) or dist (SSSP, u16)
+    unsigned long long* d_ovf = nullptr;  // SSSP lanes whose distances left 16 bits
+    HostBuf<unsigned long long> h_ovf;
     uint32_t* q[2] = {nullptr, nullptr};
     Slot* slots = nullptr;
     uint32_t* heavy_u = nullptr;
@@ -587,6 +619,7 @@ struct asc_server {
     std::deque<asc::SQuery*> pending[2];
     std::unordered_map<uint64_t, asc::SQuery*> tickets;
     uint64_t next_ticket = 1, submitted = 0, completed = 0, steps = 0, outstanding = 0, bu_steps = 0;
+    uint64_t fallbacks = 0;           // SSSP queries answered by the single-query engine
     double busy_ms = 0;
     bool stop = false;
     int error = ASC_OK;
@@ -599,7 +632,7 @@ static void engine_free(Engine& e) {
         if (p) (void)hipFree(p);
     };
     f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
-    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt);
+    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt); f(e.d_ovf);
     if (e.stream) (void)hipStreamDestroy(e.stream);
     e.stream = nullptr;
 }
@@ -621,8 +654,8 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     } else {
         ASC_HIP(hipMalloc(&e.staging, 4 * V));
     }
-    e.stride = kind == ASC_QUERY_SSSP ? (uint32_t)((lanes + 3) / 4 * 4) : (uint32_t)lanes;
-    hipError_t r = hipMalloc(&e.lanes_state, 4 * V * (size_t)e.stride);
+    e.stride = kind == ASC_QUERY_SSSP ? (uint32_t)((lanes + 7) / 8 * 8) : (uint32_t)lanes;
+    hipError_t r = hipMalloc(&e.lanes_state, (kind == ASC_QUERY_SSSP ? 2 : 4) * V * (size_t)e.stride);
     if (r != hipSuccess) {
         set_last_error("serving state of %d lanes x %llu vertices does not fit", lanes, (unsigned long long)V);
         return ASC_ERR_NOMEM;
@@ -634,6 +667,12 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     ASC_HIP(hipMalloc(&e.heavy_base, 8 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&e.s0, 4 * kMaxLanes));
     ASC_TRY(e.h_slot.alloc(1, true));
+    if (kind == ASC_QUERY_SSSP) {
+        ASC_HIP(hipMalloc(&e.d_ovf, 8));
+        ASC_HIP(hipMemsetAsync(e.d_ovf, 0, 8, e.stream));
+        ASC_TRY(e.h_ovf.alloc(1, true));
+        e.h_ovf[0] = 0;
+    }
     if (kind == ASC_QUERY_BFS) {
         ASC_HIP(hipMalloc(&e.d_cnt, 16 * kMaxLanes));
         ASC_TRY(e.h_cnt.alloc(2 * kMaxLanes, true));
@@ -651,7 +690,8 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
     a.Y = e.bits[(e.step + 1) & 1];
     a.seen = e.seen;
     a.lev = e.kind == ASC_QUERY_BFS ? e.lanes_state : nullptr;
-    a.dist = e.kind == ASC_QUERY_SSSP ? e.lanes_state : nullptr;
+    a.dist = e.kind == ASC_QUERY_SSSP ? (uint16_t*)e.lanes_state : nullptr;
+    a.ovf = e.d_ovf;
     a.q_in = e.q[e.step & 1];
     a.q_out = e.q[(e.step + 1) & 1];
     a.slots = e.slots;
@@ -719,8 +759,8 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
                 hipLaunchKernelGGL(k_lane_fill, dim3(grid), dim3(256), 0, e.stream,
                                    e.lanes_state + (size_t)q->lane * e.V, e.V, unreached);
         } else {
-            hipLaunchKernelGGL(k_reset_dist, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
-                               e.stride, mask, unreached);
+            hipLaunchKernelGGL(k_reset_dist, dim3(grid), dim3(256), 0, e.stream, (uint16_t*)e.lanes_state, e.V,
+                               e.stride, mask);
         }
         hipLaunchKernelGGL((k_seed<OffT, EdgeT>), dim3(1), dim3(kMaxLanes), 0, e.stream, a, sa, e.step, e.s0,
                            e.kind == ASC_QUERY_BFS ? 1 : 0);
@@ -748,6 +788,7 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
     }
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipMemcpyAsync(e.h_slot.p, e.slots + ((e.step + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, e.stream));
+    if (e.d_ovf) ASC_HIP(hipMemcpyAsync(e.h_ovf.p, e.d_ovf, 8, hipMemcpyDeviceToHost, e.stream));
     e.launched = true;
     return ASC_OK;
 }
@@ -766,22 +807,36 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
             e.mu[q] = e.mu[q] > e.mf[q] ? e.mu[q] - e.mf[q] : 0;
         }
     }
+    const bool zero_based = s->g->ctx->cfg.value_convention == ASC_VALUES_ZERO_BASED;
+    const uint32_t unreached = (uint32_t)(e.V + 1);
     for (int q = 0; q < e.lanes; q++) {
         SQuery* qq = e.lane_q[q];
         if (!qq) continue;
         qq->steps++;
         if (act & (1ull << q)) continue;
-        if (qq->out) {
+        const bool overflowed = e.kind == ASC_QUERY_SSSP && ((e.h_ovf[0] >> q) & 1ull);
+        if (qq->out && overflowed) {
+            // a distance left 16 bits: the single-query engine answers exactly
+            // (value convention applied there)
+            ASC_TRY(traverse(s->g, ALG_SSSP, qq->src, qq->out, nullptr));
+            s->fallbacks++;
+        } else if (qq->out) {
             if (e.kind == ASC_QUERY_BFS) {
                 ASC_HIP(hipMemcpyAsync(qq->out, e.lanes_state + (size_t)q * e.V, 4 * e.V, hipMemcpyDeviceToHost,
                                        e.stream));
             } else {
                 const int grid = s->g->ctx->num_cus * 4;
-                hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream, e.lanes_state, e.V,
-                                   e.stride, (uint32_t)q, e.staging);
+                hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream,
+                                   (const uint16_t*)e.lanes_state, e.V, e.stride, (uint32_t)q, e.staging);
                 ASC_HIP(hipMemcpyAsync(qq->out, e.staging, 4 * e.V, hipMemcpyDeviceToHost, e.stream));
             }
             ASC_HIP(hipStreamSynchronize(e.stream));
+            if (zero_based) {
+                uint32_t* o = qq->out;
+                host_pool().parallel_for(e.V, [&](uint64_t b, uint64_t en) {
+                    for (uint64_t v = b; v < en; v++) o[v] = o[v] == unreached ? 0xFFFFFFFFu : o[v] - 1;
+                });
+            }
         }
         qq->t_done = now_ms();
         e.lane_q[q] = nullptr;
@@ -903,9 +958,9 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
     uint64_t budget = o.state_budget_bytes ? o.state_budget_bytes : free_b / 2;
     budget = std::min<uint64_t>(budget, free_b > 2 * fixed ? free_b - 2 * fixed : 0);
     const bool weighted = g->h.kind == ASC_KIND_WCSR;
-    const uint64_t per_lane = 4 * V;
-    const int kinds = weighted ? 2 : 1;
-    int lanes = (int)std::min<uint64_t>((uint64_t)o.lanes, budget / kinds / per_lane);
+    // per query: u32 levels (BFS), u16 distances (SSSP)
+    const uint64_t per_lane = weighted ? 6 * V : 4 * V;
+    int lanes = (int)std::min<uint64_t>((uint64_t)o.lanes, budget / per_lane);
     if (lanes < 1) {
         set_last_error("state budget %llu bytes admits no query (%llu bytes per lane)", (unsigned long long)budget,
                        (unsigned long long)per_lane);
@@ -1017,6 +1072,7 @@ int asc_server_get_stats(asc_server* s, asc_server_stats* st) {
     st->lanes_sssp = s->eng[ASC_QUERY_SSSP].lanes;
     st->busy_ms = s->busy_ms;
     st->bottom_up_steps = s->bu_steps;
+    st->fallbacks = s->fallbacks;
     return ASC_OK;
 }
 

@@ -364,7 +364,8 @@ typedef struct {
     int lanes;                   /* max queries in flight per kind (1..64);
                                     0 = the graph's tuned width (64 untuned) */
     uint64_t state_budget_bytes; /* device bytes for per-query state (4 V bytes
-                                    per lane); 0 = half the free memory       */
+                                    per BFS lane, 2 V per SSSP lane); 0 = half
+                                    the free memory                           */
     int queue_capacity;          /* max submitted-but-unfinished queries      */
     int bfs_direction;           /* asc_bfs_direction: AUTO lets a BFS lane
                                     whose frontier is large take bottom-up
@@ -387,6 +388,8 @@ typedef struct {
     int lanes_bfs, lanes_sssp;   /* lanes granted by the memory budget        */
     double busy_ms;              /* wall time with at least one query in flight */
     uint64_t bottom_up_steps;    /* BFS engine steps with bottom-up lanes      */
+    uint64_t fallbacks;          /* SSSP queries whose distances left 16 bits,
+                                    answered by the single-query engine      */
 } asc_server_stats;
 
 /* Starts a background thread that admits and advances queries. */

@@ -89,6 +89,49 @@ def test_bottom_up_lanes_exact(wgraph):
         assert np.array_equal(res[_abi.BFS_AUTO][i], O.bfs_queue(csr, s)), s
 
 
+def test_sssp_lanes_beyond_16_bits_fall_back_exactly():
+    """Distances above 65534 do not fit a 16-bit lane: such queries are
+    answered by the single-query engine, and every result stays exact."""
+    from tests import graphs as G
+    n = 40
+    off, dst, _ = G.path_graph(n)
+    w = np.full(len(dst), 30000, np.uint32)
+    w[::3] = 7
+    off2, dst2, w2 = G.random_graph(200, 600, 21, symmetric=True, weighted=True, wmax=64)
+    # two components: a heavy path (overflows) and a light random graph
+    offs = np.concatenate([off, off2[1:] + off[-1]]).astype(np.uint64)
+    dsts = np.concatenate([dst, dst2 + n]).astype(np.uint32)
+    ws = np.concatenate([w, w2]).astype(np.uint32)
+    V = len(offs) - 1
+    csr = O.CSR(offs, dsts, ws)
+    with A.Context() as ctx, ctx.from_arrays(offs, np.stack([dsts, ws], axis=1)) as g:
+        srcs = [0, 5, n - 1, n, n + 17, n + 150]
+        outs = [np.empty(V, np.uint32) for _ in srcs]
+        with A.Server(g, lanes=8) as srv:
+            ts = [srv.submit(A.Server.SSSP, s_, o) for s_, o in zip(srcs, outs)]
+            for t in ts:
+                srv.wait(t)
+            st = srv.stats()
+        assert st["fallbacks"] >= 3
+        for s_, o in zip(srcs, outs):
+            assert np.array_equal(o, O.dijkstra(csr, s_)), s_
+
+
+def test_served_values_follow_value_convention():
+    h = A.rmat(12, 8, seed=4, weighted=True)
+    off, ew = h.offsets().copy(), h.edge_words().copy()
+    V = len(off) - 1
+    with A.Context(value_convention=_abi.VALUES_ZERO_BASED) as ctx, ctx.from_arrays(off, ew) as g:
+        outs = {k: np.empty(V, np.uint32) for k in (A.Server.BFS, A.Server.SSSP)}
+        with A.Server(g, lanes=4) as srv:
+            ts = [srv.submit(k, 3, o) for k, o in outs.items()]
+            for t in ts:
+                srv.wait(t)
+        assert np.array_equal(outs[A.Server.BFS], g.bfs(3)[0])
+        assert np.array_equal(outs[A.Server.SSSP], g.sssp(3)[0])
+        assert outs[A.Server.BFS][3] == 0 and outs[A.Server.SSSP][3] == 0
+
+
 def test_staggered_arrivals_from_threads(wgraph):
     g, off, ew = wgraph
     V = len(off) - 1
