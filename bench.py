@@ -52,7 +52,7 @@ def algorithmic_bytes(kern, s_e, s_o):
     }
 
 
-def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0):
+def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None):
     """Query serving on a resident weighted R-MAT graph: 70% BFS / 30% SSSP,
     sources uniform over vertices with degree > 0.  Reports one-query-per-
     launch throughput (plain kernels), the server's saturated throughput and
@@ -61,7 +61,8 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0):
     import threading
 
     t = time.perf_counter()
-    h = A.rmat(scale, 16, seed=seed, weighted=True, wmin=1, wmax=64)
+    if h is None:
+        h = A.rmat(scale, 16, seed=seed, weighted=True, wmin=1, wmax=64)
     V = h.num_vertices
     deg = np.diff(h.offsets())
     cand = np.nonzero(deg > 0)[0]
@@ -277,7 +278,7 @@ def main():
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
-    ap.add_argument("--serve-scale", type=int, default=22, help="R-MAT scale of the serving section (0 = skip)")
+    ap.add_argument("--serve-scale", type=int, default=26, help="R-MAT scale of the serving section (0 = skip)")
     ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
     ap.add_argument("--algo-er", type=str, default="26,30",
                     help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
@@ -432,6 +433,7 @@ def main():
         result["per_algo"] = per_algo_bench(A, _abi, O, args.algo_bfs_scale, lv, le, args.steps,
                                             args.cpu_threads or min(16, os.cpu_count() or 1))
         result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
+    hr = None
     if args.resident_scale > 0:
         t = time.perf_counter()
         hr = A.rmat(args.resident_scale, 16, seed=args.seed, weighted=True, wmin=1, wmax=64)
@@ -439,10 +441,13 @@ def main():
             f"({time.perf_counter() - t:.1f}s)")
         result["resident"] = {"workload": f"rmat_s{args.resident_scale}_ef16_w1-64_resident_u64off",
                               **resident_bench(A, _abi, hr, args.steps)}
-        hr.free()
-        del hr
+        if args.serve_scale != args.resident_scale:
+            hr.free()
+            hr = None
     if args.serve_scale > 0:
-        result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0])
+        # the serving section adopts (consumes) the resident graph when the scales match
+        result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0],
+                                          h=hr)
     print(json.dumps(result), flush=True)
     if result.get("parity", "").startswith("MISMATCH"):
         sys.exit(1)
