@@ -120,6 +120,21 @@ def read_graph(path, fmt=_abi.FMT_AUTO):
     return HostGraph(h)
 
 
+def read_edge_list(path, num_vertices=0, symmetrize=False, dedup=True, keep_self_loops=False, one_based=False,
+                   weighted=False, csc=False, wmin=1, wmax=64, seed=1, threads=0):
+    """Text / Matrix Market / .toB edge list -> HostGraph (sorted adjacency)."""
+    lib = _abi.load()
+    o = _abi.EdgeListOpts()
+    lib.asc_edge_list_opts_init(C.byref(o))
+    o.num_vertices, o.symmetrize, o.dedup, o.keep_self_loops = num_vertices, int(symmetrize), int(dedup), \
+        int(keep_self_loops)
+    o.one_based, o.weighted, o.csc, o.wmin, o.wmax, o.seed, o.threads = int(one_based), int(weighted), int(csc), \
+        wmin, wmax, seed, threads
+    h = _abi.HostCsr()
+    check(lib.asc_read_edge_list(str(path).encode(), C.byref(o), C.byref(h)), f"asc_read_edge_list({path})")
+    return HostGraph(h)
+
+
 def rmat(scale, edge_factor=16, seed=1, **kw):
     return generate(type=0, scale=scale, edge_factor=edge_factor, seed=seed, **kw)
 

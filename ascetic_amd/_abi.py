@@ -179,6 +179,23 @@ class GenParams(C.Structure):
     ]
 
 
+class EdgeListOpts(C.Structure):
+    _fields_ = [
+        ("num_vertices", C.c_uint64),
+        ("symmetrize", C.c_int),
+        ("dedup", C.c_int),
+        ("keep_self_loops", C.c_int),
+        ("one_based", C.c_int),
+        ("weighted", C.c_int),
+        ("csc", C.c_int),
+        ("wmin", C.c_uint32),
+        ("wmax", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("threads", C.c_int),
+        ("pad", C.c_int),
+    ]
+
+
 class HostCsr(C.Structure):
     _fields_ = [
         ("num_vertices", C.c_uint64),
@@ -267,6 +284,8 @@ SIGNATURES = {
     "asc_gen_params_init": (None, [C.POINTER(GenParams)]),
     "asc_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(HostCsr)]),
     "asc_read_graph": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(HostCsr)]),
+    "asc_edge_list_opts_init": (None, [C.POINTER(EdgeListOpts)]),
+    "asc_read_edge_list": (C.c_int, [C.c_char_p, C.POINTER(EdgeListOpts), C.POINTER(HostCsr)]),
     "asc_host_csr_free": (None, [C.POINTER(HostCsr)]),
     "asc_host_csr_view": (None, [C.POINTER(HostCsr), C.POINTER(CsrView)]),
     "asc_graph_adopt": (C.c_int, [P, C.POINTER(HostCsr), C.c_int, PP]),

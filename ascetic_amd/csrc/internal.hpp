@@ -115,6 +115,7 @@ int write_graph_file(const char* path, int format, const asc_csr_view* v);
 int csr_from_view(const asc_csr_view* v, HostCSR& out);
 int validate_csr(const HostCSR& h);
 int generate_graph(const asc_gen_params* p, HostCSR& out);
+int read_edge_list(const char* path, const asc_edge_list_opts* o, HostCSR& out);
 
 // ---- thread pool ------------------------------------------------------------
 class ThreadPool {

@@ -19,6 +19,11 @@
 //                           --weighted 0|1 --wmax w --csc 0|1 --out file
 //                           --format bcsr|bwcsr|bcsc|legacy32|legacy32w
 //   --type info             print the header, layout and planned split
+//   --type convert          edge list -> binary graph (converter/converter.cpp):
+//                           --input edges.txt|.mtx|.toB --out file --format
+//                           bcsr|bwcsr|bcsc|legacy32|legacy32w [--symmetrize 0|1]
+//                           [--dedup 0|1] [--self-loops 0|1] [--one-based 0|1]
+//                           [--vertices n] [--wmin w --wmax w --seed x]
 // Every run ends with one JSON line of statistics.  Errors are reported with
 // the library's message and a nonzero exit status; nothing calls exit()
 // inside the library.
@@ -54,7 +59,9 @@ void usage() {
             "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy|hybrid] [--format fmt]\n"
             "               [--values ref|zero] [--pr-mode ref|norm] [--iters n] [--out file] [--verbose 0|1]\n"
             "       ptgraph --type gen --gen rmat|er [--scale s --ef f | --vertices n --samples m] [--seed x]\n"
-            "               [--weighted 0|1] [--wmax w] [--csc 0|1] --format fmt --out file\n");
+            "               [--weighted 0|1] [--wmax w] [--csc 0|1] --format fmt --out file\n"
+            "       ptgraph --type convert --input edges.txt|.mtx|.toB --format fmt --out file [--symmetrize 0|1]\n"
+            "               [--dedup 0|1] [--self-loops 0|1] [--one-based 0|1] [--vertices n] [--wmin w --wmax w]\n");
 }
 
 int fail(int code, const char* what) {
@@ -125,6 +132,37 @@ int run_gen(const Args& a) {
     return r ? fail(r, "asc_write_graph") : 0;
 }
 
+int run_convert(const Args& a) {
+    asc_edge_list_opts o;
+    asc_edge_list_opts_init(&o);
+    const std::string input = a.get("--input"), out = a.get("--out");
+    const int fmt = parse_format(a.get("--format", "bcsr"));
+    if (input.empty() || out.empty() || fmt <= 0) {
+        usage();
+        return 2;
+    }
+    o.weighted = fmt == ASC_FMT_BWCSR || fmt == ASC_FMT_LEGACY32W;
+    o.csc = fmt == ASC_FMT_BCSC;
+    o.symmetrize = (int)a.num("--symmetrize", 0);
+    o.dedup = (int)a.num("--dedup", 1);
+    o.keep_self_loops = (int)a.num("--self-loops", 0);
+    o.one_based = (int)a.num("--one-based", 0);
+    o.num_vertices = (uint64_t)a.num("--vertices", 0);
+    o.wmin = (uint32_t)a.num("--wmin", 1);
+    o.wmax = (uint32_t)a.num("--wmax", 64);
+    o.seed = (uint64_t)a.num("--seed", 1);
+    asc_host_csr h;
+    int r = asc_read_edge_list(input.c_str(), &o, &h);
+    if (r) return fail(r, "asc_read_edge_list");
+    asc_csr_view v;
+    asc_host_csr_view(&h, &v);
+    r = asc_write_graph(out.c_str(), fmt, &v);
+    printf("{\"converted\": \"%s\", \"V\": %llu, \"E\": %llu}\n", out.c_str(),
+           (unsigned long long)h.num_vertices, (unsigned long long)h.num_edges);
+    asc_host_csr_free(&h);
+    return r ? fail(r, "asc_write_graph") : 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -137,6 +175,7 @@ int main(int argc, char** argv) {
     }
     std::string type = a.get("--type", "bfs");  // main.cu:7-9
     if (type == "gen") return run_gen(a);
+    if (type == "convert") return run_convert(a);
     std::string input = a.get("--input");
     if (input.empty()) {
         usage();

@@ -341,6 +341,32 @@ typedef struct {
 
 /* Generates a graph: self-loops and duplicate edges removed, adjacency sorted. */
 int asc_generate(const asc_gen_params* p, asc_host_csr* out);
+
+/* ---- edge-list conversion (converter/converter.cpp:22-420) -------------- */
+typedef struct {
+    uint64_t num_vertices;       /* 0: one more than the largest id           */
+    int symmetrize;              /* add v->u for every u->v                   */
+    int dedup;                   /* keep one copy of each (u, v), the lightest */
+    int keep_self_loops;
+    int one_based;               /* ids start at 1 (set by a Matrix Market
+                                    header by itself)                         */
+    int weighted;                /* {dst, w} records: the input's third column,
+                                    else w(u,v) = w(v,u) drawn in [wmin, wmax] */
+    int csc;                     /* in-CSR of sources + out-degrees (PageRank) */
+    uint32_t wmin, wmax;
+    uint64_t seed;
+    int threads;                 /* 0 = all cores                             */
+    int pad;
+} asc_edge_list_opts;
+void asc_edge_list_opts_init(asc_edge_list_opts* o);
+/* Reads a text edge list ("u v [w]" per line, separated by blanks, tabs or
+ * commas; lines starting with '#' or '%' skipped), a Matrix Market file, or
+ * binary {u32 src, u32 dst} pairs (a path ending in ".toB") into a host CSR
+ * with sorted adjacency; asc_write_graph stores it as .bcsr / .bwcsr /
+ * .bcsc.  Replaces the reference's two-step converter (text -> .toB ->
+ * binary CSR), without its defects (V from sources only, shifted offsets
+ * of trailing empty vertices, zero weights). */
+int asc_read_edge_list(const char* path, const asc_edge_list_opts* o, asc_host_csr* out);
 /* Reads a graph file into host memory only (no device needed); same
  * validation as asc_graph_load (GraphMeta.cuh:151-171). */
 int asc_read_graph(const char* path, int format, asc_host_csr* out);
