@@ -400,6 +400,9 @@ static int plan_dense_chunks(asc_graph* g, TravState* t) {
     ASC_TRY(t->h_cursor.alloc(n + 1, true));
     t->ch_v = std::move(cv);
     t->ch_e = std::move(ce);
+    // pinned host-to-device rate for choosing dense chunks
+    double h2d = 0, d2h = 0;
+    if (asc_measure_link(g->ctx, 64u << 20, 3, &h2d, &d2h) == ASC_OK && h2d > 0) t->link_bps = h2d * 1e9;
     return ASC_OK;
 }
 
@@ -507,6 +510,35 @@ static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long 
                      hi, dst);
 }
 
+// Chooses the chunks copied whole: every chunk with at least half of its
+// edges active, then, densest first, further chunks while the estimated
+// round time falls, so that the two transfer paths finish together: with S
+// the active bytes left to gather and D the bytes copied whole, a round
+// takes about max((D + S) / link, S / gather), since gathered parts also
+// cross the link.
+static void choose_dense_chunks(TravState* t, size_t EB) {
+    const uint32_t n_ch = (uint32_t)t->ch_v.size() - 1;
+    const double L = t->link_bps > 0 ? t->link_bps : 50e9, G = t->gather_bps;
+    std::vector<uint32_t> order;
+    double S = 0, D = 0;
+    for (uint32_t c = 0; c < n_ch; c++) {
+        t->h_ch_dense[c] = 0;
+        if (t->h_ch_count[c] == 0) continue;
+        S += (double)t->h_ch_edges[c] * EB;
+        if (t->ch_e[c + 1] - t->ch_e[c] <= t->part_edges) order.push_back(c);
+    }
+    auto density = [&](uint32_t c) { return (double)t->h_ch_edges[c] / (double)(t->ch_e[c + 1] - t->ch_e[c]); };
+    std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return density(x) > density(y); });
+    auto cost = [&](double s, double d) { return std::max((d + s) / L, s / G); };
+    for (uint32_t c : order) {
+        const double k = (double)t->h_ch_edges[c] * EB, C = (double)(t->ch_e[c + 1] - t->ch_e[c]) * EB;
+        if (density(c) < 0.5 && cost(S - k, D + C) >= cost(S, D)) break;
+        S -= k;
+        D += C;
+        t->h_ch_dense[c] = 1;
+    }
+}
+
 // Streams one pile: filter on the device, list to the host, u64 prefix of
 // degrees, then parts (gather + double-buffered copies) or zero-copy reads.
 // With dense chunks planned, the pile's active edges are first counted per
@@ -541,12 +573,18 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
         ASC_HIP(hipMemcpyAsync(t->h_ch_edges.p, t->d_ch_edges, 8 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipMemcpyAsync(t->h_ch_count.p, t->d_ch_count, 4 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
-        const double thr = ctx->cfg.dense_threshold > 0 ? ctx->cfg.dense_threshold : 0.5;
+        if (ctx->cfg.dense_threshold > 0) {
+            const double thr = ctx->cfg.dense_threshold;
+            for (uint32_t c = 0; c < n_ch; c++) {
+                const uint64_t ce = t->ch_e[c + 1] - t->ch_e[c];
+                t->h_ch_dense[c] = t->h_ch_count[c] > 0 && ce <= t->part_edges &&
+                                   (double)t->h_ch_edges[c] >= thr * (double)ce;
+            }
+        } else {
+            choose_dense_chunks(t, EB);
+        }
         for (uint32_t c = 0; c < n_ch; c++) {
-            const uint64_t ce = t->ch_e[c + 1] - t->ch_e[c];
-            const bool d = t->h_ch_count[c] > 0 && ce <= t->part_edges && (double)t->h_ch_edges[c] >= thr * (double)ce;
-            t->h_ch_dense[c] = d ? 1 : 0;
-            if (d) n_dense++;
+            if (t->h_ch_dense[c]) n_dense++;
             else n_sparse += t->h_ch_count[c];
         }
     }
@@ -693,6 +731,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             return ASC_OK;
         }
         // the rest: host gather into pinned staging, double-buffered copies
+        double fill_round = 0;
         const uint64_t PE = t->part_edges;
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
@@ -702,7 +741,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
             gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p);
-            rc.fill_ms += now_ms() - tg;
+            fill_round += now_ms() - tg;
             ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
             ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[sb].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_stage[sb], ctx->s_copy));
@@ -716,9 +755,12 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             ASC_TRY(launch_part(false));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
         }
+        rc.fill_ms += fill_round;
         rc.parts += nb;
         rc.streamed_edges += Ts + dense_edges;
         rc.dense_edges += dense_edges;
+        if (Ts * EB >= (32u << 20) && fill_round > 0)   // running estimate of the gather rate
+            t->gather_bps = 0.5 * t->gather_bps + 0.5 * (double)(Ts * EB) / (fill_round * 1e-3);
     }
     rc.overload_ms += now_ms() - t_begin;
     if (ctx->cfg.verbose > 1)

@@ -146,6 +146,9 @@ struct TravState {
     HostBuf<unsigned long long> h_ch_edges;
     HostBuf<uint32_t> h_ch_count, h_cursor;
     HostBuf<uint8_t> h_ch_dense;
+    // rates the dense-chunk choice balances (bytes/s): pinned host-to-device
+    // copies (measured once) and the host gather (running estimate)
+    double link_bps = 0, gather_bps = 30e9;
     // resident fast paths: frontier bitmaps, counters, sampled labels
     uint32_t* bm[2] = {nullptr, nullptr};
     DoCounters* dcnt = nullptr;
