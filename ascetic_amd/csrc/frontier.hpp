@@ -185,14 +185,28 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
     unsigned long long my_edges = 0;
 
-    for (uint64_t t0 = (uint64_t)blockIdx.x * kTile; t0 < total; t0 += (uint64_t)gridDim.x * kTile) {
+    // contiguous runs of tiles per workgroup: only the first tile searches
+    // the heavy list; later tiles start at the previous tile's last owner
+    const uint64_t n_tiles = (total + kTile - 1) / kTile;
+    const uint64_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t tb = min(n_tiles, (uint64_t)blockIdx.x * per), te = min(n_tiles, tb + per);
+    uint32_t k_last = 0;
+    for (uint64_t t = tb; t < te; t++) {
+        const uint64_t t0 = t * kTile;
+        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
         if (tid == 0) {
-            uint32_t lo = 0, hi = nh - 1;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+            uint32_t k0;
+            if (t == tb) {
+                uint32_t lo = 0, hi = nh - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (a.heavy_base[mid] <= t0) lo = mid; else hi = mid - 1;
+                }
+                k0 = lo;
+            } else {
+                k0 = k_last + (k_last + 1 < nh && a.heavy_base[k_last + 1] <= t0 ? 1u : 0u);
             }
-            s_k0 = lo;
+            s_k0 = k0;
         }
         __syncthreads();
         const uint32_t k0 = s_k0;
@@ -203,7 +217,12 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
             s_hsv[tid] = a.heavy_sv[k0 + tid];
         }
         __syncthreads();
-        const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
+        if (tid == 0) {
+            uint32_t j = 0;
+            for (uint32_t i = 1; i < nwin; i++)
+                if (s_hb[i] <= t0 + tlen - 1) j = i;
+            k_last = k0 + j;
+        }
         for (uint32_t r = w * kWave * kUnroll; r < tlen; r += kBlock * kUnroll) {
             uint32_t kk[kUnroll], vv[kUnroll], cc[kUnroll], pr[kUnroll];
             EdgeT ed[kUnroll];

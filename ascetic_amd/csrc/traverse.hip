@@ -55,29 +55,57 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
     __shared__ unsigned long long s_p[kTile];
     __shared__ uint32_t s_sv[kTile];
     __shared__ unsigned long long s_src[ZC ? kTile : 1];
-    __shared__ uint32_t s_k[2];
+    __shared__ uint32_t s_k;
     ASC_STAGE_DECL;
 
     Slot* nxt = a.slots + ((level + 1) & 3);
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
     unsigned long long my_edges = 0, my_verts = 0;
 
-    for (uint64_t t0 = p.lo + (uint64_t)blockIdx.x * kTile; t0 < p.hi; t0 += (uint64_t)gridDim.x * kTile) {
+    // each workgroup takes a contiguous run of tiles: only its first tile
+    // searches the prefix for its first owner; a later tile's first owner is
+    // the previous tile's last owner or the entry after it (no entry is empty)
+    const uint64_t n_tiles = (p.hi - p.lo + kTile - 1) / kTile;
+    const uint64_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
+    const uint64_t tb = min(n_tiles, (uint64_t)blockIdx.x * per), te = min(n_tiles, tb + per);
+    uint32_t k_last = 0;
+    for (uint64_t t = tb; t < te; t++) {
+        const uint64_t t0 = p.lo + t * kTile;
         const uint64_t t1 = min(t0 + (uint64_t)kTile, p.hi);
-        if (tid < 2) {
-            const uint64_t key = tid == 0 ? t0 : t1 - 1;
-            uint32_t lo = 0, hi = p.n - 1;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi + 1) >> 1;
-                if (p.prefix[mid] <= key) lo = mid; else hi = mid - 1;
+        if (tid == 0) {
+            uint32_t k0;
+            if (t == tb) {
+                uint32_t lo = 0, hi = p.n - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (p.prefix[mid] <= t0) lo = mid; else hi = mid - 1;
+                }
+                k0 = lo;
+            } else {
+                k0 = k_last + (k_last + 1 < p.n && p.prefix[k_last + 1] <= t0 ? 1u : 0u);
             }
-            s_k[tid] = lo;
+            s_k = k0;
         }
         __syncthreads();
-        const uint32_t k0 = s_k[0];
-        const uint32_t m = s_k[1] - k0 + 1;
+        const uint32_t k0 = s_k;
+        // owners of the tile: the window entries whose start is <= t1 - 1 (a
+        // prefix of the window, at most kTile long), loaded 256 at a time
+        const uint32_t avail = min((uint32_t)kTile, p.n - k0);
+        uint32_t m = 0;
+        for (uint32_t base = 0; base < avail; base += kBlock) {
+            const uint32_t j = base + tid;
+            bool in = false;
+            if (j < avail) {
+                const unsigned long long pj = p.prefix[k0 + j];
+                in = pj <= t1 - 1;
+                if (in) s_p[j] = pj;
+            }
+            const uint32_t c = (uint32_t)__syncthreads_count(in);
+            m += c;
+            if (c < kBlock) break;
+        }
+        k_last = k0 + m - 1;
         for (uint32_t j = tid; j < m; j += kBlock) {
-            s_p[j] = p.prefix[k0 + j];
             s_sv[j] = (ALGO == ALG_BFS) ? level + 2 : list_sv[k0 + j];
             if (ZC) s_src[j] = (unsigned long long)a.off[p.list[k0 + j]] - p.tail_base;
         }
