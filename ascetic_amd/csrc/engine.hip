@@ -137,7 +137,7 @@ int graph_upload(asc_graph* g) {
     // symmetry decides the resident fast paths (direction-optimizing BFS,
     // union-find CC); checked on the device copy when every edge is on it
     g->symmetric = -1;
-    if (g->static_vertices == V && g->h.kind != ASC_KIND_CSC) ASC_TRY(check_symmetric(g));
+    if (g->static_vertices == V) ASC_TRY(check_symmetric(g));
     return ASC_OK;
 }
 

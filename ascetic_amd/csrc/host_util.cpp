@@ -141,6 +141,7 @@ void asc_pr_opts_init(asc_pr_opts* o) {
     o->iterations = 20;
     o->threshold = 0.001;
     o->max_iterations = 0;
+    o->method = ASC_PR_AUTO;
 }
 
 void asc_gen_params_init(asc_gen_params* p) {

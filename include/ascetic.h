@@ -275,7 +275,17 @@ typedef struct {
     int iterations;              /* normalised mode: fixed count (20)         */
     double threshold;            /* reference mode: 0.001                     */
     int max_iterations;          /* reference mode safety cap (0 = 10000)     */
+    int method;                  /* normalised mode: asc_pr_method            */
 } asc_pr_opts;
+
+/* Normalised PageRank schedule (asc_pr_opts.method). */
+typedef enum {
+    ASC_PR_AUTO = 0,     /* the pull (the faster one measured on MI355X)     */
+    ASC_PR_PULL = 1,     /* G-lane pull over the in-CSR                      */
+    ASC_PR_BLOCKED = 2   /* propagation-blocked (scatter to slots by
+                            destination bin, LDS sums per bin); needs every
+                            edge resident, else ASC_ERR_UNSUPPORTED         */
+} asc_pr_method;
 void asc_pr_opts_init(asc_pr_opts* o);
 
 /* rank_f32: V floats (normalised mode); rank_f64: V doubles (reference mode);

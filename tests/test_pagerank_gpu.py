@@ -51,6 +51,51 @@ def test_pr_normalised_hub_and_dangling(gpu_ctx):
     assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
 
 
+@pytest.mark.parametrize("case", ["sym", "dir", "hub", "rmat", "multi_bin"])
+@pytest.mark.parametrize("method", [_abi.PR_PULL, _abi.PR_BLOCKED])
+def test_pr_blocked_and_pull_agree(gpu_ctx, case, method):
+    """Propagation-blocked and pulled normalised PageRank both meet the L1
+    bound: symmetric graphs (out-CSR = in-CSR), directed ones (transposed on
+    the device), heavy destinations (kept on the FP64 pull), R-MAT and a
+    graph spanning several 32768-vertex bins."""
+    if case in ("sym", "dir"):
+        V = 3000
+        off, dst, _ = G.random_graph(V, 20000, 31, symmetric=case == "sym")
+    elif case == "hub":
+        V = 9000
+        edges = [(i, 0) for i in range(1, 6000)] + [(0, i) for i in range(1, 70)] + [(i, i + 1) for i in range(1, 4000)]
+        off, dst, _ = G.csr_from_edges(V, edges)
+    elif case == "rmat":
+        h = A.rmat(13, 16, seed=8)
+        off, dst = h.offsets().copy(), h.dst().copy()
+        V = len(off) - 1
+    else:
+        V = 100000
+        off, dst, _ = G.random_graph(V, 400000, 32, symmetric=False)
+    ioff, isrc, od = csc_of(off, dst, V)
+    ref, ref_res = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 20)
+    with gpu_ctx.from_arrays(ioff, isrc, out_degree=od) as g:
+        r, res, st = g.pagerank(iterations=20, method=method)
+        r2, _, _ = g.pagerank(iterations=20, method=method)
+    assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1, case
+    assert np.abs(r2.astype(np.float64) - ref).sum() <= TOL_L1, case
+    assert np.allclose(res, ref_res, rtol=1e-3, atol=1e-7)
+
+
+def test_pr_blocked_needs_resident_graph():
+    off, dst, _ = G.random_graph(3000, 12000, 33, symmetric=True)
+    ioff, isrc, od = csc_of(off, dst, 3000)
+    E = int(ioff[-1])
+    with A.Context(edge_budget_bytes=4 * (E // 3), part_bytes=4 * 4096) as ctx, \
+            ctx.from_arrays(ioff, isrc, out_degree=od) as g:
+        with pytest.raises(A.AscError) as e:
+            g.pagerank(iterations=5, method=_abi.PR_BLOCKED)
+        assert e.value.code == _abi.ERR_UNSUPPORTED
+        ref, _ = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 5)
+        r, _, _ = g.pagerank(iterations=5)   # AUTO: the streamed pull
+        assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
+
+
 def test_pr_rmat_csc(gpu_ctx):
     h = A.rmat(14, 16, seed=4, csc=True)
     ioff, isrc, od = h.offsets().copy(), h.dst().copy(), h.out_degree().copy()
