@@ -216,8 +216,22 @@ __global__ __launch_bounds__(kBlock) void k_pr_pull(PrArgs a) {
         const OffT b = off[v], e = off[v + 1];
         const bool heavy = a.skip_heavy && (uint64_t)(e - b) >= a.heavy_thr;
         double s = 0;
-        if (!heavy)
-            for (OffT k = b + sub; k < e; k += G) s += (double)a.contrib_in[src[k]];
+        if (!heavy) {
+            // four edges per lane per pass: their source ids, then their
+            // contributions, are loaded back to back (a lane past the end
+            // re-reads the list's first edge and drops it)
+            for (OffT k = b + sub; k < e; k += 4 * G) {
+                uint32_t u[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) u[j] = src[k + (OffT)(j * G) < e ? k + (OffT)(j * G) : b];
+                float c[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) c[j] = a.contrib_in[u[j]];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (k + (OffT)(j * G) < e) s += (double)c[j];
+            }
+        }
 #pragma unroll
         for (int d = G / 2; d > 0; d >>= 1) s += __shfl_xor(s, d, G);
         if (sub == 0 && !heavy) pr_finish_vertex(a, v, s, base, res, dang);
