@@ -507,8 +507,8 @@ int effective_heavy_threshold(const asc_graph* g) {
     return (int)thr;
 }
 
-// SSSP bucket width: configured, else twice the mean edge weight (computed
-// once per graph); a negative setting disables the far pile.
+// SSSP bucket width: configured, else a multiple of the mean edge weight
+// (computed once per graph); a negative setting disables the far pile.
 static uint32_t sssp_delta(asc_graph* g) {
     if (g->sssp_delta > 0) return (uint32_t)g->sssp_delta;
     if (g->sssp_delta < 0) return 0;
@@ -522,7 +522,11 @@ static uint32_t sssp_delta(asc_graph* g) {
         });
         g->mean_weight = E ? (double)sum.load() / (double)E : 1.0;
     }
-    return std::max<uint32_t>(1, (uint32_t)(2.0 * g->mean_weight));
+    // streamed graphs take wide bands (each band re-streams the on-demand
+    // vertices it improves, so fewer rounds win); fully resident ones narrow
+    // bands (fewer re-expansions; a step costs only a launch and a sync)
+    const double f = g->static_vertices == g->h.V ? 0.25 : 2.0;
+    return std::max<uint32_t>(1, (uint32_t)(f * g->mean_weight));
 }
 
 // Copies the adjacency of list entries overlapping [lo, hi) of the
