@@ -56,6 +56,8 @@ class Config(C.Structure):
         ("bfs_direction", C.c_int),
         ("cc_method", C.c_int),
         ("dense_threshold", C.c_double),
+        ("stream_overlap", C.c_int),
+        ("pad_cfg", C.c_int),
     ]
 
 

@@ -132,6 +132,8 @@ void asc_config_init(asc_config* c) {
     c->bfs_direction = ASC_BFS_AUTO;
     c->cc_method = ASC_CC_AUTO;
     c->dense_threshold = 0.0;
+    c->stream_overlap = 1;
+    c->pad_cfg = 0;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

@@ -13,7 +13,12 @@
 //   --pr-mode ref|norm      PageRank: reference active-set FP64 (default) or
 //                           normalised fixed iterations (--iters, default 20)
 //   --out <file>            write the V result values (raw little-endian)
-//   --verbose 0|1           per-phase timings in the reference's wording
+//   --verbose 0|1|2         per-phase timings in the reference's wording (2: per round)
+//   --dense f               dense-chunk fraction (0 = per round, >1 = gather only)
+//   --overlap 0|1           resident steps between streamed chunks (SSSP, CC)
+//   --bfs td|auto           top-down only, or direction-optimizing when possible
+//   --cc lp|auto            label propagation only, or union-find when possible
+//   --pr-method auto|pull|blocked
 //   --type gen              write a synthetic graph: --gen rmat|er --scale s
 //                           --ef f --vertices n --samples m --seed x
 //                           --weighted 0|1 --wmax w --csc 0|1 --out file
@@ -57,7 +62,8 @@ void usage() {
     fprintf(stderr,
             "usage: ptgraph --input <graph file> [--type bfs|cc|sssp|pr|info] [--source N] [--adviseK f]\n"
             "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy|hybrid] [--format fmt]\n"
-            "               [--values ref|zero] [--pr-mode ref|norm] [--iters n] [--out file] [--verbose 0|1]\n"
+            "               [--values ref|zero] [--pr-mode ref|norm] [--pr-method auto|pull|blocked] [--iters n]\n"
+            "               [--out file] [--verbose 0|1|2] [--dense f] [--overlap 0|1] [--bfs td|auto] [--cc lp|auto]\n"
             "       ptgraph --type gen --gen rmat|er [--scale s --ef f | --vertices n --samples m] [--seed x]\n"
             "               [--weighted 0|1] [--wmax w] [--csc 0|1] --format fmt --out file\n"
             "       ptgraph --type convert --input edges.txt|.mtx|.toB --format fmt --out file [--symmetrize 0|1]\n"
@@ -194,6 +200,10 @@ int main(int argc, char** argv) {
     cfg.part_bytes = (uint64_t)a.num("--part-mb", 0) * (1ull << 20);
     const std::string sm = a.get("--stream", "gather");
     cfg.stream_mode = sm == "zerocopy" ? ASC_STREAM_ZEROCOPY : sm == "hybrid" ? ASC_STREAM_HYBRID : ASC_STREAM_GATHER;
+    cfg.dense_threshold = a.num("--dense", 0.0);
+    cfg.stream_overlap = (int)a.num("--overlap", 1);
+    cfg.bfs_direction = a.get("--bfs", "auto") == "td" ? ASC_BFS_TOP_DOWN : ASC_BFS_AUTO;
+    cfg.cc_method = a.get("--cc", "auto") == "lp" ? ASC_CC_LABEL_PROP : ASC_CC_AUTO;
     cfg.value_convention = a.get("--values", "ref") == "zero" ? ASC_VALUES_ZERO_BASED : ASC_VALUES_REFERENCE;
     cfg.verbose = (int)a.num("--verbose", 0);
     cfg.profile = 1;
@@ -244,6 +254,8 @@ int main(int argc, char** argv) {
         asc_pr_opts_init(&o);
         const bool norm = a.get("--pr-mode", "ref") == "norm";
         o.mode = norm ? ASC_PR_NORMALISED : ASC_PR_REFERENCE;
+        const std::string pm = a.get("--pr-method", "auto");
+        o.method = pm == "pull" ? ASC_PR_PULL : pm == "blocked" ? ASC_PR_BLOCKED : ASC_PR_AUTO;
         o.iterations = (int)a.num("--iters", 20);
         if (norm) {
             std::vector<float> rank(gi.num_vertices);

@@ -578,8 +578,8 @@ static void choose_dense_chunks(TravState* t, size_t EB) {
 // edges is copied whole by DMA from the pinned tail (no host gather) and its
 // active vertices, kept in their own list segment, are expanded from it.
 template <int ALGO, class OffT, class EdgeT>
-static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, const uint32_t* pile, uint32_t npile,
-                       int grid, RunCounters& rc) {
+static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, const uint32_t* pile,
+                       uint32_t npile, int grid, RunCounters& rc) {
     asc_ctx* ctx = g->ctx;
     TravState* t = g->trav;
     const bool prof = ctx->cfg.profile != 0;
@@ -708,6 +708,24 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
                                    a, level, p, list_sv);
         });
     };
+    // SSSP/CC with stream_overlap: after each part or chunk, one resident step
+    // expands what it pushed while the next transfer is in flight.  Each part
+    // and each step takes its own level, so a resident vertex improved again
+    // after its step is queued again (stamps are per level); BFS keeps one
+    // level per round because its values are levels.
+    const bool overlap = ALGO != ALG_BFS && ctx->cfg.stream_overlap != 0;
+    auto resident_step = [&]() -> int {
+        if (!overlap) return ASC_OK;
+        level++;
+        a.q_in = t->q[level & 1];
+        a.q_out = t->q[(level + 1) & 1];
+        ASC_TRY(timed_launch(t, prof, KERN_LOCAL, s, [&] {
+            hipLaunchKernelGGL((k_expand_local<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        }));
+        return timed_launch(t, prof, KERN_HEAVY, s, [&] {
+            hipLaunchKernelGGL((k_expand_heavy<ALGO, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
+        });
+    };
     uint64_t dense_edges = 0;
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
         void* dptr = nullptr;
@@ -740,6 +758,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             p.hi = t->h_prefix[t->h_cursor[c] + t->h_ch_count[c]];
             if (p.hi > p.lo) ASC_TRY(launch_part(true));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
+            ASC_TRY(resident_step());
             dense_edges += e1 - e0;
         }
         const uint64_t Ts = t->h_prefix[n_sparse];
@@ -786,6 +805,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t level, c
             p.hi = hi;
             ASC_TRY(launch_part(false));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
+            ASC_TRY(resident_step());
         }
         rc.fill_ms += fill_round;
         rc.parts += nb;

@@ -144,8 +144,14 @@ typedef struct {
                                     vertex range of at most one part) whose
                                     active edges reach this fraction of its
                                     edges is copied whole by DMA from the
-                                    pinned tail instead of gathered; 0 = 0.5,
-                                    > 1 = never                              */
+                                    pinned tail instead of gathered; 0 = chosen
+                                    per round (at least half active, then by a
+                                    link/gather balance), > 1 = never        */
+    int stream_overlap;          /* SSSP/CC: 1 (default) = expand the resident
+                                    frontier after every streamed part or
+                                    chunk, while the next one is in flight;
+                                    0 = after the whole round                */
+    int pad_cfg;
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;

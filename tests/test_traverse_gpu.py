@@ -142,6 +142,28 @@ def test_dense_chunk_copies(name, mk, thr):
         assert dense > 0
 
 
+@pytest.mark.parametrize("thr", [1e-9, 2.0])
+@pytest.mark.parametrize("name,mk", CASES[3:])
+def test_stream_overlap_exact(name, mk, thr):
+    """Resident steps between streamed chunks / parts (SSSP, CC) give the
+    oracle's values, with whole-chunk copies and with gathering only."""
+    say("overlap", name, "threshold", thr)
+    off, dst, _ = mk()
+    w = weights_for(off, 8)
+    csr = O.CSR(off, dst, w)
+    E = int(off[-1])
+    with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 2048, dense_threshold=thr,
+                   stream_overlap=True) as ctx:
+        with make_graph(ctx, off, dst, w) as g:
+            for s in (0, int(np.argmax(np.diff(off))), len(off) - 2):
+                d, st = g.sssp(s)
+                assert np.array_equal(d, O.dijkstra(csr, s)), (name, s)
+            lab, _ = g.cc()
+            assert np.array_equal(lab, O.cc_minreach(csr)), name
+            lv, _ = g.bfs(0)
+            assert np.array_equal(lv, O.bfs_queue(csr, 0)), name
+
+
 def test_everything_on_demand(gpu_ctx):
     off, dst, _ = G.random_graph(3000, 12000, 9, symmetric=False)
     w = weights_for(off, 9)
