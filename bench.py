@@ -277,6 +277,8 @@ def main():
     ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stream-baselines", dest="stream_baselines", action="store_false",
+                    help="skip the partition-streaming and gather-only comparison runs")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
     ap.add_argument("--serve-scale", type=int, default=26, help="R-MAT scale of the serving section (0 = skip)")
     ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
@@ -404,6 +406,30 @@ def main():
             "overload_ms_per_step": sum(r["overload_ms"] for r in runs) / args.steps,
         },
     }
+
+    # comparison points for the on-demand transfer (same graph and source):
+    # every chunk holding an active vertex copied whole (partition streaming,
+    # the reference's bfs.cu / sssp.cu scheme at 64 MiB partitions), per-vertex
+    # gathering only (the CalculateOpt.cu scheme), both at the same budget, and
+    # no device edges at all with kernels reading the graph from host memory
+    # (the unified-memory "Share" drivers of bfs.cu / sssp.cu in the form they
+    # take without page migration)
+    if args.stream_baselines:
+        result["stream_baselines"] = {}
+        for name, thr, bud, md in (("partition_streaming", 1e-9, budget, mode), ("gather_only", 2.0, budget, mode),
+                                   ("share_zero_copy", 0.0, 1, _abi.STREAM_ZEROCOPY)):
+            with A.Context(edge_budget_bytes=bud, part_bytes=args.part_mb << 20, stream_mode=md,
+                           sssp_delta=args.delta, dense_threshold=thr) as c2, c2.from_arrays(off, ew) as g2:
+                g2.sssp(src, output=False)
+                rs = [g2.sssp(src, output=False)[1] for _ in range(max(2, args.steps // 2))]
+                ms = float(np.median([r["total_ms"] for r in rs]))
+                sb = float(np.median([r["streamed_bytes"] for r in rs]))
+                result["stream_baselines"][name] = {
+                    "ms_per_step": round(ms, 3), "gteps": round(reached_deg / 2 / (ms / 1e3) / 1e9, 4),
+                    "streamed_GB_per_step": round(sb / 1e9, 3), "link_GBps": round(sb / (ms / 1e3) / 1e9, 2)}
+        result["stream_baselines"]["default"] = {
+            "ms_per_step": round(step_s * 1e3, 3), "gteps": round(gteps, 4),
+            "streamed_GB_per_step": round(streamed / 1e9, 3), "link_GBps": round(streamed / step_s / 1e9, 2)}
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O

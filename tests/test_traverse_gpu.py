@@ -164,11 +164,15 @@ def test_stream_overlap_exact(name, mk, thr):
             assert np.array_equal(lv, O.bfs_queue(csr, 0)), name
 
 
-def test_everything_on_demand(gpu_ctx):
+@pytest.mark.parametrize("mode", [_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY])
+def test_everything_on_demand(gpu_ctx, mode):
+    """A device budget of one byte: every edge is on demand. In zero-copy mode
+    kernels read the whole graph from host memory at the link, the form the
+    reference's unified-memory ("Share") drivers take without page migration."""
     off, dst, _ = G.random_graph(3000, 12000, 9, symmetric=False)
     w = weights_for(off, 9)
     csr = O.CSR(off, dst, w)
-    with A.Context(edge_budget_bytes=1, part_bytes=8 * 1024) as ctx:
+    with A.Context(edge_budget_bytes=1, part_bytes=8 * 1024, stream_mode=mode) as ctx:
         with make_graph(ctx, off, dst, w) as g:
             assert g.info()["static_vertices"] <= 1
             d, st = g.sssp(5)
@@ -176,6 +180,8 @@ def test_everything_on_demand(gpu_ctx):
             assert st["parts"] >= 1
             lab, _ = g.cc()
             assert np.array_equal(lab, O.cc_minreach(csr))
+            lv, _ = g.bfs(5)
+            assert np.array_equal(lv, O.bfs_queue(csr, 5))
 
 
 def test_value_conventions():
