@@ -264,41 +264,8 @@ def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--scale", type=int, default=24)
-    ap.add_argument("--edge-factor", type=int, default=16)
-    ap.add_argument("--seed", type=int, default=24)
-    ap.add_argument("--budget-frac", type=float, default=0.4)
-    ap.add_argument("--part-mb", type=int, default=64)
-    ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
-    ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-stream-baselines", dest="stream_baselines", action="store_false",
-                    help="skip the partition-streaming and gather-only comparison runs")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
-    ap.add_argument("--serve-scale", type=int, default=26, help="R-MAT scale of the serving section (0 = skip)")
-    ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
-    ap.add_argument("--algo-er", type=str, default="26,30",
-                    help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
-    ap.add_argument("--serve-queries", type=int, default=512)
-    ap.add_argument("--resident-scale", type=int, default=26,
-                    help="R-MAT scale of the resident BFS/SSSP/CC section (0 = skip)")
-    ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="also launch one plain device copy of this many MiB (counter calibration)")
-    args = ap.parse_args()
-
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one MI355X only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the traversal would shard); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
-    import ascetic_amd as A
-    from ascetic_amd import _abi
-
+def stream_section(A, _abi, args):
+    """The main line: SSSP on the budget-capped R-MAT graph, streamed."""
     t = time.perf_counter()
     h = A.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=True, wmin=1, wmax=64)
     off = h.offsets()
@@ -453,12 +420,57 @@ def main():
     g.close()
     ctx.close()
     del h, off, ew
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=24, help="R-MAT scale of the main streamed line (0 = skip)")
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=24)
+    ap.add_argument("--budget-frac", type=float, default=0.4)
+    ap.add_argument("--part-mb", type=int, default=64)
+    ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
+    ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-stream-baselines", dest="stream_baselines", action="store_false",
+                    help="skip the partition-streaming and gather-only comparison runs")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--serve-scale", type=int, default=26, help="R-MAT scale of the serving section (0 = skip)")
+    ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
+    ap.add_argument("--algo-er", type=str, default="26,30",
+                    help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
+    ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--resident-scale", type=int, default=26,
+                    help="R-MAT scale of the resident BFS/SSSP/CC section (0 = skip)")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also launch one plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one MI355X only: this build has no multi-GPU path "
+              "(see DESIGN.md for how the traversal would shard); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import ascetic_amd as A
+    from ascetic_amd import _abi
+
+    if args.scale > 0:
+        result = stream_section(A, _abi, args)
+    else:   # sections only (profiling one workload at a time)
+        result = {"metric": METRIC, "value": None, "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
+                  "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                  "dtype": "u32", "data": "synthetic", "config": {"workload": "sections only"}}
     if args.algo_bfs_scale > 0 and args.algo_er:
         from oracle import oracle as O
         lv, le = (int(x) for x in args.algo_er.split(","))
         result["per_algo"] = per_algo_bench(A, _abi, O, args.algo_bfs_scale, lv, le, args.steps,
                                             args.cpu_threads or min(16, os.cpu_count() or 1))
-        result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
+        if args.scale > 0:
+            result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
     hr = None
     if args.resident_scale > 0:
         t = time.perf_counter()
