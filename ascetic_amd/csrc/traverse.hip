@@ -46,12 +46,12 @@ namespace asc {
 
 // Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
 // the streamed list (list[k] owns [prefix[k], prefix[k+1]), no zero-degree
-// entries; sv[k] is the value list[k] is expanded with).  ZC = false: edges
+// entries).  SSSP/CC expand each entry with its value when its tile starts,
+// BFS with the level.  ZC = false: edges
 // come from the device part buffer holding [lo, hi).  ZC = true: from mapped
 // pinned host memory (tail = edges from static_edges on).
 template <int ALGO, class OffT, class EdgeT, bool ZC, bool PK>
-__global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p,
-                                                         const uint32_t* list_sv) {
+__global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p) {
     __shared__ unsigned long long s_p[kTile];
     __shared__ uint32_t s_sv[kTile];
     __shared__ unsigned long long s_src[ZC ? kTile : 1];
@@ -106,8 +106,18 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
         }
         k_last = k0 + m - 1;
         for (uint32_t j = tid; j < m; j += kBlock) {
-            s_sv[j] = (ALGO == ALG_BFS) ? level + 2 : list_sv[k0 + j];
-            if (ZC) s_src[j] = (unsigned long long)a.off[p.list[k0 + j]] - p.tail_base;
+            const uint32_t u = p.list[k0 + j];
+            if (ALGO == ALG_BFS) {
+                s_sv[j] = level + 2;
+            } else {
+                // the current value (it may have improved since the pile was
+                // filtered, by earlier chunks or resident steps); done[u] ends
+                // the round at the largest value any tile expanded u with
+                const uint32_t cur = a.val[u];
+                s_sv[j] = cur;
+                if (a.done[u] < cur) atomicMax(a.done + u, cur);
+            }
+            if (ZC) s_src[j] = (unsigned long long)a.off[u] - p.tail_base;
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)(t1 - t0);
@@ -157,8 +167,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
 }
 
 // Compacts a pile before it is streamed: drops zero-degree vertices and (SSSP,
-// CC) vertices already expanded at their current value, and snapshots the
-// value each kept vertex will be expanded with.
+// CC) vertices already expanded at their current value.  A kept vertex's
+// `done` is set to 0 for the round: each tile of k_expand_part that expands it
+// reads its value at that moment and raises `done` to it, so `done` ends the
+// round at the largest (weakest) value any of its edges was relaxed with, and
+// the next round re-expands it only if its value is lower than that.
 // Chunk of on-demand vertex v: the largest i < n_ch with ch_v[i] <= v.
 __device__ __forceinline__ uint32_t chunk_of(const uint32_t* ch_v, uint32_t n_ch, uint32_t v) {
     uint32_t lo = 0, hi = n_ch - 1;
@@ -172,7 +185,7 @@ __device__ __forceinline__ uint32_t chunk_of(const uint32_t* ch_v, uint32_t n_ch
 template <int ALGO, class OffT>
 __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const uint32_t* pile, uint32_t n,
                                                         uint32_t* val, uint32_t* done, uint32_t* out,
-                                                        uint32_t* out_sv, uint32_t* out_deg, uint32_t* count) {
+                                                        uint32_t* out_deg, uint32_t* count) {
     const uint32_t lane = lane_id();
     for (uint32_t base = (blockIdx.x * kBlock) & ~(kWave - 1u); base < n; base += gridDim.x * kBlock) {
         const uint32_t i = base + threadIdx.x;
@@ -186,7 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const u
             if (ALGO != ALG_BFS && keep) {
                 d = val[v];
                 keep = done[v] != d;
-                if (keep) done[v] = d;
+                if (keep) done[v] = 0;   // the part kernels raise it to the values they expand v with
             }
         }
         const uint64_t m = wave_ballot(keep);
@@ -198,7 +211,6 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile(const OffT* off, const u
             const uint32_t pos = at + lanes_below(m);
             out[pos] = v;
             out_deg[pos] = deg;
-            if (ALGO != ALG_BFS) out_sv[pos] = d;
         }
     }
 }
@@ -243,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void k_pile_chunks(const OffT* off, const u
 template <int ALGO, class OffT>
 __global__ __launch_bounds__(kBlock) void k_filter_pile_chunked(const OffT* off, const uint32_t* pile, uint32_t n,
                                                                 uint32_t* val, uint32_t* done, uint32_t* out,
-                                                                uint32_t* out_sv, uint32_t* out_deg,
+                                                                uint32_t* out_deg,
                                                                 const uint32_t* ch_v, uint32_t n_ch,
                                                                 const uint8_t* ch_dense, uint32_t* cursor) {
     __shared__ uint32_t s_cnt[kMaxChunks + 1];
@@ -262,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile_chunked(const OffT* off,
             if (ALGO != ALG_BFS && keep) {
                 d = val[v];
                 keep = done[v] != d;
-                if (keep) done[v] = d;
+                if (keep) done[v] = 0;   // as in k_filter_pile
             }
             if (keep) {
                 const uint32_t c = chunk_of(ch_v, n_ch, v);
@@ -278,7 +290,6 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile_chunked(const OffT* off,
             const uint32_t pos = s_base[seg] + rank;
             out[pos] = v;
             out_deg[pos] = deg;
-            if (ALGO != ALG_BFS) out_sv[pos] = d;
         }
         __syncthreads();
     }
@@ -462,9 +473,8 @@ int trav_ensure(asc_graph* g) {
         const uint64_t nod = V - g->static_vertices;
         ASC_HIP(hipMalloc(&t->pend[0], 4 * nod));
         ASC_HIP(hipMalloc(&t->pend[1], 4 * nod));
-        // od_list holds the filtered list, then the snapshot values, then the
-        // degrees (nod entries each)
-        ASC_HIP(hipMalloc(&t->od_list, 12 * nod));
+        // od_list holds the filtered list, then the degrees (nod entries each)
+        ASC_HIP(hipMalloc(&t->od_list, 8 * nod));
         ASC_HIP(hipMalloc(&t->od_prefix, 8 * (nod + 1)));
         ASC_TRY(t->h_list.alloc(2 * nod, true));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
@@ -523,9 +533,11 @@ static uint32_t sssp_delta(asc_graph* g) {
         g->mean_weight = E ? (double)sum.load() / (double)E : 1.0;
     }
     // streamed graphs take wide bands (each band re-streams the on-demand
-    // vertices it improves, so fewer rounds win); fully resident ones narrow
-    // bands (fewer re-expansions; a step costs only a launch and a sync)
-    const double f = g->static_vertices == g->h.V ? 0.25 : 2.0;
+    // vertices it improves, so fewer rounds win; streamed vertices are
+    // expanded with their value at expansion time, which limits the
+    // re-expansions wide bands cause); fully resident ones narrow bands (fewer
+    // re-expansions; a step costs only a launch and a sync)
+    const double f = g->static_vertices == g->h.V ? 0.25 : 6.0;
     return std::max<uint32_t>(1, (uint32_t)(f * g->mean_weight));
 }
 
@@ -586,8 +598,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     hipStream_t s = ctx->s_main;
     const double t_begin = now_ms();
     const uint64_t nod = g->h.V - g->static_vertices;
-    uint32_t* list_sv = t->od_list + nod;
-    uint32_t* list_deg = t->od_list + 2 * nod;
+    uint32_t* list_deg = t->od_list + nod;
     const bool pk = g->packed;
     const size_t EB = pk ? 4 : g->h.edge_bytes();
     const uint64_t S = g->static_edges;
@@ -632,7 +643,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_HIP(hipMemcpyAsync(t->d_ch_dense, t->h_ch_dense.p, n_ch, hipMemcpyHostToDevice, s));
         ASC_HIP(hipMemcpyAsync(t->d_cursor, t->h_cursor.p, 4 * ((size_t)n_ch + 1), hipMemcpyHostToDevice, s));
         hipLaunchKernelGGL((k_filter_pile_chunked<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile,
-                           npile, a.val, a.done, t->od_list, list_sv, list_deg, t->d_ch_v, n_ch,
+                           npile, a.val, a.done, t->od_list, list_deg, t->d_ch_v, n_ch,
                            (const uint8_t*)t->d_ch_dense, t->d_cursor);
         ASC_HIP(hipGetLastError());
         ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
@@ -641,7 +652,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     } else {
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
         hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile, npile,
-                           a.val, a.done, t->od_list, list_sv, list_deg, &t->piles->n_list);
+                           a.val, a.done, t->od_list, list_deg, &t->piles->n_list);
         ASC_HIP(hipGetLastError());
         // count, kept list and degrees (computed on the device) in one round
         // trip: the kept count is at most the pile size, so copy that many
@@ -696,16 +707,16 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             const int grid = pgrid;
             if (zc && pk)
                 hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, true>), dim3(grid), dim3(kBlock), 0, s, a,
-                                   level, p, list_sv);
+                                   level, p);
             else if (zc)
                 hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, false>), dim3(grid), dim3(kBlock), 0, s, a,
-                                   level, p, list_sv);
+                                   level, p);
             else if (pk)
                 hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, true>), dim3(grid), dim3(kBlock), 0, s,
-                                   a, level, p, list_sv);
+                                   a, level, p);
             else
                 hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, false>), dim3(grid), dim3(kBlock), 0, s,
-                                   a, level, p, list_sv);
+                                   a, level, p);
         });
     };
     // SSSP/CC with stream_overlap: after each part or chunk, one resident step
