@@ -376,12 +376,14 @@ class Server:
     BFS = _abi.QUERY_BFS
     SSSP = _abi.QUERY_SSSP
 
-    def __init__(self, graph, lanes=64, state_budget_bytes=0, queue_capacity=4096, bfs_direction=_abi.BFS_AUTO):
+    def __init__(self, graph, lanes=64, state_budget_bytes=0, queue_capacity=4096, bfs_direction=_abi.BFS_AUTO,
+                 step_graphs=True):
         lib = _abi.load()
         o = _abi.ServerOpts()
         lib.asc_server_opts_init(C.byref(o))
         o.lanes, o.state_budget_bytes, o.queue_capacity = lanes, state_budget_bytes, queue_capacity
         o.bfs_direction = bfs_direction
+        o.step_graphs = int(bool(step_graphs))
         h = C.c_void_p()
         check(lib.asc_server_create(graph._h, C.byref(o), C.byref(h)), "asc_server_create")
         self._h = h

@@ -217,6 +217,8 @@ class ServerOpts(C.Structure):
         ("state_budget_bytes", C.c_uint64),
         ("queue_capacity", C.c_int),
         ("bfs_direction", C.c_int),
+        ("step_graphs", C.c_int),
+        ("pad", C.c_int),
     ]
 
 
@@ -230,6 +232,7 @@ class ServerStats(C.Structure):
         ("busy_ms", C.c_double),
         ("bottom_up_steps", C.c_uint64),
         ("fallbacks", C.c_uint64),
+        ("graph_steps", C.c_uint64),
     ]
 
 

@@ -171,6 +171,8 @@ void asc_server_opts_init(asc_server_opts* o) {
     o->state_budget_bytes = 0;
     o->queue_capacity = 4096;
     o->bfs_direction = ASC_BFS_AUTO;
+    o->step_graphs = 1;
+    o->pad = 0;
 }
 
 }  // extern "C"

@@ -50,6 +50,15 @@ struct SeedArgs {
     uint32_t src[kMaxLanes];
 };
 
+// What changes from one engine step to the next, read by the step's kernels
+// from device memory (written by a copy at the head of the step), so that one
+// recorded launch sequence serves every step of the same parity.
+struct StepParams {
+    uint32_t step;                   // engine step number
+    uint32_t pad;
+    unsigned long long bu;           // BFS lanes that take a bottom-up step
+};
+
 template <class OffT, class EdgeT>
 struct BatchArgs {
     const OffT* off;
@@ -67,7 +76,7 @@ struct BatchArgs {
     unsigned long long* heavy_bits;
     unsigned long long* heavy_base;
     const uint32_t* s0;              // step at which each lane was seeded
-    unsigned long long td_mask;      // BFS lanes expanded top-down this step
+    const StepParams* sp;
     uint64_t V;
     uint32_t stride;                 // SSSP row length (lanes)
     uint32_t heavy_thr;
@@ -92,7 +101,9 @@ __device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uin
 }
 
 template <class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a, uint32_t step) {
+__global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a) {
+    const uint32_t step = a.sp->step;
+    const unsigned long long td_mask = ~a.sp->bu;   // lanes expanded top-down
     __shared__ uint32_t s_pref[kBlock];
     __shared__ OffT s_beg[kBlock];
     __shared__ unsigned long long s_bits[kBlock];
@@ -117,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a
         unsigned long long bits = 0;
         if (i < n) {
             const uint32_t u = a.q_in[i];
-            bits = a.X[u] & a.td_mask;   // bottom-up lanes' bits were consumed already
+            bits = a.X[u] & td_mask;   // bottom-up lanes' bits were consumed already
             a.X[u] = 0;
             if (bits) {
                 beg = a.off[u];
@@ -163,7 +174,8 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a
 }
 
 template <class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a, uint32_t step) {
+__global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a) {
+    const uint32_t step = a.sp->step;
     constexpr int kWin = 8;
     __shared__ unsigned long long s_hb[kWin], s_hbits[kWin];
     __shared__ OffT s_hoff[kWin];
@@ -220,13 +232,16 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a
     if (lane == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
-// One bottom-up step for the lanes in bu_mask (symmetric graph: in-neighbours
-// are the adjacency).  Thread per vertex; four neighbour ids are loaded
-// before their frontier words are probed.
+// One bottom-up step for the lanes in the step's bottom-up mask (symmetric
+// graph: in-neighbours are the adjacency); returns at once when it is empty.
+// Thread per vertex; four neighbour ids are loaded before their frontier
+// words are probed.
 template <class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, EdgeT> a, uint32_t step,
-                                                            unsigned long long bu_mask) {
+__global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, EdgeT> a) {
     __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    const uint32_t step = a.sp->step;
+    const unsigned long long bu_mask = a.sp->bu;
+    if (!bu_mask) return;
     Slot* nxt = a.slots + ((step + 1) & 3);
     WaveStage<kStageCap> st{s_stage[wave_id()], 0};
     unsigned long long act = 0;
@@ -267,15 +282,17 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, Edge
     if (lane_id() == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
-// Per lane q: vertices with bit q in Y (cnt[q]) and, for the lanes in
-// `edge_mask`, their summed degrees (cnt[64 + q]), over the queue q of the
-// vertices whose Y word is non-zero (count in slot->n_res).  A wave takes 64
+// Per lane q: vertices with bit q in Y (cnt[q]) and, for the top-down lanes
+// of the step, their summed degrees (cnt[64 + q]), over the queue q of the
+// vertices whose Y word is non-zero (count in the next step's slot).  A wave takes 64
 // queue entries; for each bit present in the wave a ballot counts the
 // vertices, and for the masked bits a wave sum adds the degrees.
 template <class OffT>
 __global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const unsigned long long* Y, const uint32_t* q,
-                                                        const Slot* slot, unsigned long long edge_mask,
+                                                        const Slot* slots, const StepParams* sp,
                                                         unsigned long long* cnt) {
+    const Slot* slot = slots + ((sp->step + 1) & 3);
+    const unsigned long long edge_mask = ~sp->bu;
     __shared__ unsigned long long s_n[kMaxLanes], s_m[kMaxLanes];
     const uint32_t lane = lane_id();
     if (threadIdx.x < kMaxLanes) {
@@ -383,7 +400,8 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
 }
 
 template <class OffT>
-__global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a, uint32_t step) {
+__global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a) {
+    const uint32_t step = a.sp->step;
     __shared__ uint32_t s_pref[kBlock];
     __shared__ OffT s_beg[kBlock];
     __shared__ uint32_t s_u[kBlock];
@@ -455,7 +473,8 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a,
 }
 
 template <class OffT>
-__global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a, uint32_t step) {
+__global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a) {
+    const uint32_t step = a.sp->step;
     constexpr int kWin = 8;
     __shared__ unsigned long long s_hb[kWin], s_hbits[kWin];
     __shared__ OffT s_hoff[kWin];
@@ -604,6 +623,16 @@ struct Engine {
     uint64_t nf[kMaxLanes] = {}, prev_nf[kMaxLanes] = {}, mf[kMaxLanes] = {}, mu[kMaxLanes] = {};
     unsigned long long* d_cnt = nullptr;
     HostBuf<unsigned long long> h_cnt;
+    // the step's parameters (device copy written at the head of each step)
+    StepParams* d_sp = nullptr;
+    HostBuf<StepParams> h_sp;
+    // recorded step sequences, one per step parity (the frontier buffers
+    // alternate), and what they were recorded with
+    bool use_graphs = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    int graph_grid[2] = {0, 0};
+    uint32_t graph_heavy[2] = {0, 0};
+    uint64_t graph_steps = 0;         // steps run by replaying a recorded sequence
 };
 
 }  // namespace asc
@@ -619,6 +648,7 @@ struct asc_server {
     std::unordered_map<uint64_t, asc::SQuery*> tickets;
     uint64_t next_ticket = 1, submitted = 0, completed = 0, steps = 0, outstanding = 0, bu_steps = 0;
     uint64_t fallbacks = 0;           // SSSP queries answered by the single-query engine
+    uint64_t graph_steps = 0;         // engine steps replayed from a recorded hipGraph
     double busy_ms = 0;
     bool stop = false;
     int error = ASC_OK;
@@ -631,7 +661,11 @@ static void engine_free(Engine& e) {
         if (p) (void)hipFree(p);
     };
     f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
-    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt); f(e.d_ovf);
+    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt); f(e.d_ovf); f(e.d_sp);
+    for (hipGraphExec_t& x : e.graph) {
+        if (x) (void)hipGraphExecDestroy(x);
+        x = nullptr;
+    }
     if (e.stream) (void)hipStreamDestroy(e.stream);
     e.stream = nullptr;
 }
@@ -665,7 +699,9 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     ASC_HIP(hipMalloc(&e.heavy_bits, 8 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&e.heavy_base, 8 * (size_t)kHeavyCap));
     ASC_HIP(hipMalloc(&e.s0, 4 * kMaxLanes));
-    ASC_TRY(e.h_slot.alloc(1, true));
+    ASC_TRY(e.h_slot.alloc(4, true));
+    ASC_HIP(hipMalloc(&e.d_sp, sizeof(StepParams)));
+    ASC_TRY(e.h_sp.alloc(1, true));
     if (kind == ASC_QUERY_SSSP) {
         ASC_HIP(hipMalloc(&e.d_ovf, 8));
         ASC_HIP(hipMemsetAsync(e.d_ovf, 0, 8, e.stream));
@@ -698,7 +734,7 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
     a.heavy_bits = e.heavy_bits;
     a.heavy_base = e.heavy_base;
     a.s0 = e.s0;
-    a.td_mask = ~e.bu;
+    a.sp = e.d_sp;
     a.V = e.V;
     a.stride = e.stride;
     a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
@@ -736,6 +772,40 @@ static void choose_directions(asc_graph* g, Engine& e, const std::vector<SQuery*
     e.bu &= e.busy;
 }
 
+// The launch sequence of one engine step: the step's parameters are copied
+// to the device, the step's kernels run, and the slots, the lane counts (BFS
+// with bottom-up lanes) and the overflow word (SSSP) are copied back.  The
+// same sequence is launched directly or recorded into a hipGraph; it holds no
+// memset node (counters are cleared by a kernel).
+template <class OffT, class EdgeT>
+static int record_step(asc_graph* g, Engine& e, const BatchArgs<OffT, EdgeT>& a, int grid) {
+    (void)g;
+    ASC_HIP(hipMemcpyAsync(e.d_sp, e.h_sp.p, sizeof(StepParams), hipMemcpyHostToDevice, e.stream));
+    if constexpr (std::is_same<EdgeT, WEdge>::value) {
+        if (e.kind == ASC_QUERY_SSSP) {
+            hipLaunchKernelGGL((k_mssp_local<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a);
+            hipLaunchKernelGGL((k_mssp_heavy<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a);
+        }
+    }
+    if (e.kind == ASC_QUERY_BFS) {
+        if (e.bu_ok)
+            hipLaunchKernelGGL((k_msbfs_bottom_up<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a);
+        hipLaunchKernelGGL((k_msbfs_local<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a);
+        hipLaunchKernelGGL((k_msbfs_heavy<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a);
+        if (e.bu_ok) {
+            hipLaunchKernelGGL(k_lane_fill, dim3(1), dim3(256), 0, e.stream, (uint32_t*)e.d_cnt,
+                               (uint64_t)(4 * kMaxLanes), 0u);
+            hipLaunchKernelGGL((k_lane_counts<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a.off, a.Y, a.q_out,
+                               e.slots, e.d_sp, e.d_cnt);
+            ASC_HIP(hipMemcpyAsync(e.h_cnt.p, e.d_cnt, 16 * kMaxLanes, hipMemcpyDeviceToHost, e.stream));
+        }
+    }
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipMemcpyAsync(e.h_slot.p, e.slots, 4 * sizeof(Slot), hipMemcpyDeviceToHost, e.stream));
+    if (e.d_ovf) ASC_HIP(hipMemcpyAsync(e.h_ovf.p, e.d_ovf, 8, hipMemcpyDeviceToHost, e.stream));
+    return ASC_OK;
+}
+
 // Resets and seeds newly admitted lanes, then launches one step.
 template <class OffT, class EdgeT>
 static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
@@ -765,29 +835,38 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
                            e.kind == ASC_QUERY_BFS ? 1 : 0);
         ASC_HIP(hipGetLastError());
     }
-    if constexpr (std::is_same<EdgeT, WEdge>::value) {
-        if (e.kind == ASC_QUERY_SSSP) {
-            hipLaunchKernelGGL((k_mssp_local<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
-            hipLaunchKernelGGL((k_mssp_heavy<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
+    e.h_sp[0].step = e.step;
+    e.h_sp[0].pad = 0;
+    e.h_sp[0].bu = e.kind == ASC_QUERY_BFS ? e.bu : 0;
+    if (e.kind == ASC_QUERY_BFS) e.launched_bu = e.bu != 0;
+    if (!e.use_graphs) {
+        const int rr = record_step<OffT, EdgeT>(g, e, a, grid);
+        if (rr != ASC_OK) return rr;
+    } else {
+        const int par = (int)(e.step & 1);
+        if (!e.graph[par] || e.graph_grid[par] != grid || e.graph_heavy[par] != a.heavy_thr) {
+            // record the step's launch sequence once per parity (again when
+            // the launch width or the heavy threshold was retuned)
+            if (e.graph[par]) ASC_HIP(hipGraphExecDestroy(e.graph[par]));
+            e.graph[par] = nullptr;
+            hipGraph_t gr = nullptr;
+            ASC_HIP(hipStreamBeginCapture(e.stream, hipStreamCaptureModeThreadLocal));
+            const int rr = record_step<OffT, EdgeT>(g, e, a, grid);
+            const hipError_t ec = hipStreamEndCapture(e.stream, &gr);
+            if (rr != ASC_OK) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return rr;
+            }
+            ASC_HIP(ec);
+            const hipError_t ei = hipGraphInstantiate(&e.graph[par], gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            ASC_HIP(ei);
+            e.graph_grid[par] = grid;
+            e.graph_heavy[par] = a.heavy_thr;
         }
+        ASC_HIP(hipGraphLaunch(e.graph[par], e.stream));
+        e.graph_steps++;
     }
-    if (e.kind == ASC_QUERY_BFS) {
-        e.launched_bu = e.bu != 0;
-        if (e.bu)
-            hipLaunchKernelGGL((k_msbfs_bottom_up<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step,
-                               e.bu);
-        hipLaunchKernelGGL((k_msbfs_local<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
-        hipLaunchKernelGGL((k_msbfs_heavy<OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, e.stream, a, e.step);
-        if (e.bu_ok) {
-            ASC_HIP(hipMemsetAsync(e.d_cnt, 0, 16 * kMaxLanes, e.stream));
-            hipLaunchKernelGGL((k_lane_counts<OffT>), dim3(grid), dim3(kBlock), 0, e.stream, a.off, a.Y, a.q_out,
-                               e.slots + ((e.step + 1) & 3), ~e.bu, e.d_cnt);
-            ASC_HIP(hipMemcpyAsync(e.h_cnt.p, e.d_cnt, 16 * kMaxLanes, hipMemcpyDeviceToHost, e.stream));
-        }
-    }
-    ASC_HIP(hipGetLastError());
-    ASC_HIP(hipMemcpyAsync(e.h_slot.p, e.slots + ((e.step + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, e.stream));
-    if (e.d_ovf) ASC_HIP(hipMemcpyAsync(e.h_ovf.p, e.d_ovf, 8, hipMemcpyDeviceToHost, e.stream));
     e.launched = true;
     return ASC_OK;
 }
@@ -797,7 +876,7 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
     if (!e.launched) return ASC_OK;
     ASC_HIP(hipStreamSynchronize(e.stream));
     e.launched = false;
-    const unsigned long long act = e.h_slot[0].pad[0];
+    const unsigned long long act = e.h_slot[(e.step + 1) & 3].pad[0];
     if (e.kind == ASC_QUERY_BFS && e.bu_ok) {
         for (int q = 0; q < e.lanes; q++) {
             e.prev_nf[q] = e.nf[q];
@@ -910,6 +989,7 @@ static void server_loop(asc_server* s) {
             s->busy_ms += now_ms() - t_step;
             s->steps++;
             if (s->eng[ASC_QUERY_BFS].launched_bu) s->bu_steps++;
+            s->graph_steps = s->eng[0].graph_steps + s->eng[1].graph_steps;
             if (r != ASC_OK) {
                 // a device error fails every query in flight or waiting
                 s->error = r;
@@ -972,6 +1052,7 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
     s->opts = o;
     int r = engine_init(s->eng[ASC_QUERY_BFS], ASC_QUERY_BFS, lanes, V);
     s->eng[ASC_QUERY_BFS].bu_ok = o.bfs_direction == ASC_BFS_AUTO && resident_symmetric(g);
+    s->eng[0].use_graphs = s->eng[1].use_graphs = o.step_graphs != 0;
     if (r == ASC_OK) r = engine_init(s->eng[ASC_QUERY_SSSP], ASC_QUERY_SSSP, weighted ? lanes : 0, V);
     if (r != ASC_OK) {
         engine_free(s->eng[0]);
@@ -1073,6 +1154,7 @@ int asc_server_get_stats(asc_server* s, asc_server_stats* st) {
     st->busy_ms = s->busy_ms;
     st->bottom_up_steps = s->bu_steps;
     st->fallbacks = s->fallbacks;
+    st->graph_steps = s->graph_steps;
     return ASC_OK;
 }
 

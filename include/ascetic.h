@@ -412,6 +412,10 @@ typedef struct {
     int bfs_direction;           /* asc_bfs_direction: AUTO lets a BFS lane
                                     whose frontier is large take bottom-up
                                     steps on a symmetric graph               */
+    int step_graphs;             /* 1 (default): each engine step replays the
+                                    step's launch sequence recorded once into
+                                    a hipGraph; 0: launches every kernel      */
+    int pad;
 } asc_server_opts;
 void asc_server_opts_init(asc_server_opts* o);
 
@@ -432,6 +436,7 @@ typedef struct {
     uint64_t bottom_up_steps;    /* BFS engine steps with bottom-up lanes      */
     uint64_t fallbacks;          /* SSSP queries whose distances left 16 bits,
                                     answered by the single-query engine      */
+    uint64_t graph_steps;        /* engine steps replayed from a hipGraph     */
 } asc_server_stats;
 
 /* Starts a background thread that admits and advances queries. */

@@ -57,6 +57,43 @@ def test_mix_equals_alone(wgraph):
         assert np.array_equal(o, ref), (k, s)
 
 
+def test_step_graphs_replay_exactly(wgraph):
+    """Engine steps replayed from recorded hipGraphs (the default) give the
+    same results as steps launched kernel by kernel: many replays of both
+    parities, with admissions (eager seeding kernels) and host syncs between
+    them, every result compared."""
+    g, off, ew = wgraph
+    V = len(off) - 1
+    deg = np.diff(off)
+    rng = np.random.default_rng(13)
+    cand = np.nonzero(deg > 0)[0]
+    srcs = [int(x) for x in rng.choice(cand, 120, replace=False)]
+    kinds = [A.Server.SSSP if i % 3 == 0 else A.Server.BFS for i in range(len(srcs))]
+    res = {}
+    for graphs in (True, False):
+        outs = [np.empty(V, np.uint32) for _ in srcs]
+        say("serving with step graphs", graphs)
+        with A.Server(g, lanes=32, step_graphs=graphs) as srv:
+            # two waves, so that lanes are admitted while others are in flight
+            tickets = [srv.submit(k, s, o) for k, s, o in zip(kinds[:60], srcs[:60], outs[:60])]
+            for t in tickets[:20]:
+                srv.wait(t)
+            tickets += [srv.submit(k, s, o) for k, s, o in zip(kinds[60:], srcs[60:], outs[60:])]
+            for t in tickets[20:]:
+                srv.wait(t)
+            st = srv.stats()
+        if graphs:
+            assert st["graph_steps"] >= 8 and st["graph_steps"] >= st["steps"] // 2
+        else:
+            assert st["graph_steps"] == 0
+        res[graphs] = outs
+    for i, (k, s) in enumerate(zip(kinds, srcs)):
+        assert np.array_equal(res[True][i], res[False][i]), (k, s)
+    for i, (k, s) in enumerate(list(zip(kinds, srcs))[:30]):
+        alone, _ = (g.sssp(s) if k == A.Server.SSSP else g.bfs(s))
+        assert np.array_equal(res[True][i], alone), (k, s)
+
+
 def test_bottom_up_lanes_exact(wgraph):
     """BFS lanes that switch to bottom-up steps (symmetric graph) give the
     same levels as top-down-only serving, single runs and the oracle."""
