@@ -295,6 +295,69 @@ __global__ __launch_bounds__(kBlock) void k_filter_pile_chunked(const OffT* off,
     }
 }
 
+// Exclusive u64 prefix of a filtered list's u32 degrees, on the device (the
+// part kernels search it): tile sums, one workgroup scanning the tile sums,
+// then each tile rescanned from its base.  out[n] = the total.
+constexpr uint32_t kScanPer = 8;
+constexpr uint32_t kScanTile = kBlock * kScanPer;
+
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const uint32_t* x, uint32_t n, unsigned long long* tile_sum) {
+    __shared__ unsigned long long s_w[kWavesPerBlock];
+    const uint64_t b = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+    unsigned long long acc = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; j++)
+        if (b + j < n) acc += x[b + j];
+    acc = wave_sum(acc);
+    if (lane_id() == 0) s_w[wave_id()] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long t = 0;
+#pragma unroll
+        for (int i = 0; i < kWavesPerBlock; i++) t += s_w[i];
+        tile_sum[blockIdx.x] = t;
+    }
+}
+
+// One workgroup: exclusive scan of nt tile sums in place; out_total = sum.
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(unsigned long long* tile_sum, uint32_t nt,
+                                                           unsigned long long* out_total) {
+    __shared__ unsigned long long s_tmp[kWavesPerBlock];
+    const uint32_t per = (nt + kBlock - 1) / kBlock;
+    const uint32_t b = threadIdx.x * per, e = min(nt, b + per);
+    unsigned long long acc = 0;
+    for (uint32_t i = b; i < e; i++) acc += tile_sum[i];
+    unsigned long long total;
+    unsigned long long base = block_scan_excl(acc, s_tmp, total);
+    for (uint32_t i = b; i < e; i++) {
+        const unsigned long long v = tile_sum[i];
+        tile_sum[i] = base;
+        base += v;
+    }
+    if (threadIdx.x == 0) *out_total = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* x, uint32_t n,
+                                                       const unsigned long long* tile_base, unsigned long long* out) {
+    __shared__ unsigned long long s_tmp[kWavesPerBlock];
+    const uint64_t b = (uint64_t)blockIdx.x * kScanTile + (uint64_t)threadIdx.x * kScanPer;
+    uint32_t v[kScanPer];
+    unsigned long long acc = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; j++) {
+        v[j] = b + j < n ? x[b + j] : 0u;
+        acc += v[j];
+    }
+    unsigned long long total;
+    unsigned long long run = tile_base[blockIdx.x] + block_scan_excl(acc, s_tmp, total);
+#pragma unroll
+    for (uint32_t j = 0; j < kScanPer; j++)
+        if (b + j < n) {
+            out[b + j] = run;
+            run += v[j];
+        }
+}
+
 // Moves the far pile's entries whose value fell below theta into the current
 // frontier (resident) or the on-demand pile; the rest stays far.
 template <class OffT, class EdgeT>
@@ -387,7 +450,7 @@ void trav_free(TravState* t) {
         f(t->q[i]); f(t->pend[i]); f(t->far[i]); f(t->part[i]);
     }
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
-    f(t->od_list); f(t->od_prefix); f(t->slots); f(t->piles); f(t->stats);
+    f(t->od_list); f(t->od_prefix); f(t->scan_tmp); f(t->slots); f(t->piles); f(t->stats);
     f(t->bm[0]); f(t->bm[1]); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
     f(t->d_cursor);
     for (int i = 0; i < 2; i++) {
@@ -434,6 +497,7 @@ static int plan_dense_chunks(asc_graph* g, TravState* t) {
     ASC_HIP(hipMalloc(&t->d_ch_dense, n));
     ASC_HIP(hipMalloc(&t->d_cursor, 4 * (n + 1)));
     ASC_TRY(t->h_ch_edges.alloc(n, true));
+    t->h_ch_lo.assign(n, 0);
     ASC_TRY(t->h_ch_count.alloc(n, true));
     ASC_TRY(t->h_ch_dense.alloc(n, true));
     ASC_TRY(t->h_cursor.alloc(n + 1, true));
@@ -476,6 +540,7 @@ int trav_ensure(asc_graph* g) {
         // od_list holds the filtered list, then the degrees (nod entries each)
         ASC_HIP(hipMalloc(&t->od_list, 8 * nod));
         ASC_HIP(hipMalloc(&t->od_prefix, 8 * (nod + 1)));
+        ASC_HIP(hipMalloc(&t->scan_tmp, 8 * (nod / kScanTile + 2)));
         ASC_TRY(t->h_list.alloc(2 * nod, true));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
@@ -589,6 +654,18 @@ static void choose_dense_chunks(TravState* t, size_t EB) {
 // chunk; a chunk whose active edges reach asc_config.dense_threshold of its
 // edges is copied whole by DMA from the pinned tail (no host gather) and its
 // active vertices, kept in their own list segment, are expanded from it.
+// od_prefix[0..n] = exclusive u64 prefix of the n degrees (device only).
+static int scan_degrees(TravState* t, const uint32_t* deg, uint32_t n, hipStream_t s) {
+    if (n == 0) return ASC_OK;
+    const uint32_t nt = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(nt), dim3(kBlock), 0, s, deg, n, t->scan_tmp);
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, t->scan_tmp, nt, t->od_prefix + n);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nt), dim3(kBlock), 0, s, deg, n, (const unsigned long long*)t->scan_tmp,
+                       t->od_prefix);
+    ASC_HIP(hipGetLastError());
+    return ASC_OK;
+}
+
 template <int ALGO, class OffT, class EdgeT>
 static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, const uint32_t* pile,
                        uint32_t npile, int grid, RunCounters& rc) {
@@ -646,8 +723,12 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                            npile, a.val, a.done, t->od_list, list_deg, t->d_ch_v, n_ch,
                            (const uint8_t*)t->d_ch_dense, t->d_cursor);
         ASC_HIP(hipGetLastError());
-        ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
-        ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)m, hipMemcpyDeviceToHost, s));
+        ASC_TRY(scan_degrees(t, list_deg, m, s));
+        // the host gathers the sparse segment only: it alone comes back
+        if (n_sparse) {
+            ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
+            ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
+        }
         ASC_HIP(hipStreamSynchronize(s));
     } else {
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
@@ -662,36 +743,46 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_HIP(hipStreamSynchronize(s));
         m = t->h_piles[0].n_list;
         n_sparse = m;
+        if (m) ASC_TRY(scan_degrees(t, list_deg, m, s));
     }
     if (m == 0) {
         rc.pre_ms += now_ms() - t_begin;
         return ASC_OK;
     }
     const uint32_t* h_deg = t->h_list.p + nod;
-    // u64 exclusive prefix of the degrees, in parallel
+    // u64 exclusive prefix of the sparse segment's degrees (the host gather's
+    // index; the same values as the device prefix), in parallel
     const int nt = ctx->pool->size();
     std::vector<uint64_t> esum(nt + 1, 0);
-    const uint64_t chunk = (m + nt - 1) / nt;
+    const uint64_t chunk = (n_sparse + nt - 1) / nt;
     ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = 0;
+        uint64_t b = std::min<uint64_t>(n_sparse, tt * chunk), e = std::min<uint64_t>(n_sparse, b + chunk), acc = 0;
         for (uint64_t i = b; i < e; i++) acc += h_deg[i];
         esum[tt + 1] = acc;
     });
     for (int i = 0; i < nt; i++) esum[i + 1] += esum[i];
     ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(m, tt * chunk), e = std::min<uint64_t>(m, b + chunk), acc = esum[tt];
+        uint64_t b = std::min<uint64_t>(n_sparse, tt * chunk), e = std::min<uint64_t>(n_sparse, b + chunk);
+        uint64_t acc = esum[tt];
         for (uint64_t i = b; i < e; i++) {
             t->h_prefix[i] = acc;
             acc += h_deg[i];
         }
     });
-    const uint64_t T = esum[nt];
-    t->h_prefix[m] = T;
+    const uint64_t Ts = esum[nt];
+    t->h_prefix[n_sparse] = Ts;
+    // dense segments follow in chunk order; each holds its chunk's active
+    // edges, so their bounds in the prefix follow from the chunk counts
+    uint64_t T = Ts;
+    for (uint32_t c = 0; c < n_ch && n_dense; c++)
+        if (t->h_ch_dense[c]) {
+            t->h_ch_lo[c] = T;
+            T += t->h_ch_edges[c];
+        }
     const double t_pre = now_ms();
     rc.pre_ms += t_pre - t_begin;
     rc.node_sum += m;
     rc.stream_rounds++;
-    ASC_HIP(hipMemcpyAsync(t->od_prefix, t->h_prefix.p, 8 * ((size_t)m + 1), hipMemcpyHostToDevice, s));
 
     PartArgs<EdgeT> p{};
     p.list = t->od_list;
@@ -765,14 +856,13 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             p.zc = (const EdgeT*)t->part[b];
             p.zc32 = (const uint32_t*)t->part[b];
             p.tail_base = e0;
-            p.lo = t->h_prefix[t->h_cursor[c]];
-            p.hi = t->h_prefix[t->h_cursor[c] + t->h_ch_count[c]];
+            p.lo = t->h_ch_lo[c];
+            p.hi = t->h_ch_lo[c] + t->h_ch_edges[c];
             if (p.hi > p.lo) ASC_TRY(launch_part(true));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
             dense_edges += e1 - e0;
         }
-        const uint64_t Ts = t->h_prefix[n_sparse];
         if (ctx->cfg.stream_mode == ASC_STREAM_HYBRID) {
             // the rest read by the kernel from the mapped pinned tail
             if (Ts > 0) {

@@ -123,6 +123,7 @@ struct TravState {
     unsigned long long* heavy_base = nullptr;
     uint32_t* od_list = nullptr;
     unsigned long long* od_prefix = nullptr;
+    unsigned long long* scan_tmp = nullptr;   // tile sums of the device prefix scan
     Slot* slots = nullptr;
     Piles* piles = nullptr;
     DevStats* stats = nullptr;
@@ -144,6 +145,7 @@ struct TravState {
     uint8_t* d_ch_dense = nullptr;              // chunk copied whole this round
     uint32_t* d_cursor = nullptr;               // list segment cursors (n_ch + 1)
     HostBuf<unsigned long long> h_ch_edges;
+    std::vector<uint64_t> h_ch_lo;  // start of each dense chunk's segment in the list's edge prefix
     HostBuf<uint32_t> h_ch_count, h_cursor;
     HostBuf<uint8_t> h_ch_dense;
     // rates the dense-chunk choice balances (bytes/s): pinned host-to-device
