@@ -447,17 +447,19 @@ void trav_free(TravState* t) {
     };
     f(t->val); f(t->stamp); f(t->pstamp); f(t->fstamp); f(t->done); f(t->visited);
     for (int i = 0; i < 2; i++) {
-        f(t->q[i]); f(t->pend[i]); f(t->far[i]); f(t->part[i]);
+        f(t->q[i]); f(t->pend[i]); f(t->far[i]);
     }
+    for (int i = 0; i < kPartBufs; i++) f(t->part[i]);
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
     f(t->od_list); f(t->od_prefix); f(t->scan_tmp); f(t->slots); f(t->piles); f(t->stats);
     f(t->bm[0]); f(t->bm[1]); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
     f(t->d_cursor);
-    for (int i = 0; i < 2; i++) {
+    for (int i = 0; i < kPartBufs; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
         if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
-        if (t->ev_stage[i]) (void)hipEventDestroy(t->ev_stage[i]);
     }
+    for (int i = 0; i < 2; i++)
+        if (t->ev_stage[i]) (void)hipEventDestroy(t->ev_stage[i]);
     if (t->ev_meta) (void)hipEventDestroy(t->ev_meta);
     for (auto e : t->ev_pool) (void)hipEventDestroy(e);
     delete t;
@@ -544,17 +546,19 @@ int trav_ensure(asc_graph* g) {
         ASC_TRY(t->h_list.alloc(2 * nod, true));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
-        for (int i = 0; i < 2; i++) {
-            if (g->ctx->cfg.stream_mode != ASC_STREAM_ZEROCOPY) {
+        for (int i = 0; i < kPartBufs; i++) {
+            t->pref_chunk[i] = -1;
+            if (g->ctx->cfg.stream_mode != ASC_STREAM_ZEROCOPY)
                 ASC_HIP(hipMalloc(&t->part[i], EB * t->part_edges));
-                if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER)
-                    ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
-            }
             ASC_HIP(hipEventCreateWithFlags(&t->ev_copy[i], hipEventDisableTiming));
             ASC_HIP(hipEventCreateWithFlags(&t->ev_kern[i], hipEventDisableTiming));
-            ASC_HIP(hipEventCreateWithFlags(&t->ev_stage[i], hipEventDisableTiming));
             // recorded once so that every copy can wait for its buffer's last reader
             ASC_HIP(hipEventRecord(t->ev_kern[i], g->ctx->s_main));
+        }
+        for (int i = 0; i < 2; i++) {
+            if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER)
+                ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
+            ASC_HIP(hipEventCreateWithFlags(&t->ev_stage[i], hipEventDisableTiming));
         }
         ASC_HIP(hipEventCreateWithFlags(&t->ev_meta, hipEventDisableTiming));
         ASC_TRY(plan_dense_chunks(g, t));
@@ -654,6 +658,33 @@ static void choose_dense_chunks(TravState* t, size_t EB) {
 // chunk; a chunk whose active edges reach asc_config.dense_threshold of its
 // edges is copied whole by DMA from the pinned tail (no host gather) and its
 // active vertices, kept in their own list segment, are expanded from it.
+// At the start of a run that may stream, the first chunks of the on-demand
+// region are copied into the part buffers while the resident part is
+// relaxed (the link is idle until the first streaming round).  The first
+// round uses a copy when its dense chunks come in that order (chunk b in part
+// buffer b, the round-robin order), else the buffer is simply overwritten.
+static int prefetch_chunks(asc_graph* g, TravState* t) {
+    asc_ctx* ctx = g->ctx;
+    for (int b = 0; b < kPartBufs; b++) t->pref_chunk[b] = -1;
+    const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
+    if (!n_ch || ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || ctx->cfg.dense_threshold > 1.0) return ASC_OK;
+    const bool pk = g->packed;
+    const char* tail = pk ? (const char*)g->ptail.p : (const char*)g->zc_tail.p;
+    if (!tail) return ASC_OK;
+    const size_t EB = pk ? 4 : g->h.edge_bytes();
+    const uint64_t S = g->static_edges;
+    for (int b = 0; b < kPartBufs && (uint32_t)b < n_ch; b++) {
+        const uint64_t e0 = t->ch_e[b], e1 = t->ch_e[b + 1];
+        if (e1 - e0 > t->part_edges) break;
+        ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
+        ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
+                               ctx->s_copy));
+        ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+        t->pref_chunk[b] = b;
+    }
+    return ASC_OK;
+}
+
 // od_prefix[0..n] = exclusive u64 prefix of the n degrees (device only).
 static int scan_degrees(TravState* t, const uint32_t* deg, uint32_t n, hipStream_t s) {
     if (n == 0) return ASC_OK;
@@ -846,12 +877,15 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         const char* tail = pk ? (const char*)g->ptail.p : (const char*)g->zc_tail.p;
         for (uint32_t c = 0; c < n_ch && n_dense; c++) {
             if (!t->h_ch_dense[c]) continue;
-            const int b = (int)(nb++ & 1);
+            const int b = (int)(nb++ % kPartBufs);
             const uint64_t e0 = t->ch_e[c], e1 = t->ch_e[c + 1];
-            ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
-            ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
-                                   ctx->s_copy));
-            ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+            if (t->pref_chunk[b] != (int)c) {   // not copied ahead at the start of the run
+                ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
+                ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
+                                       ctx->s_copy));
+                ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+            }
+            t->pref_chunk[b] = -1;
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.zc = (const EdgeT*)t->part[b];
             p.zc32 = (const uint32_t*)t->part[b];
@@ -888,7 +922,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
             const int sb = (int)(q & 1);
-            const int b = (int)(nb++ & 1);
+            const int b = (int)(nb++ % kPartBufs);
             const uint64_t lo = q * PE, hi = std::min(Ts, lo + PE);
             if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
@@ -915,11 +949,12 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         if (Ts * EB >= (32u << 20) && fill_round > 0)   // running estimate of the gather rate
             t->gather_bps = 0.5 * t->gather_bps + 0.5 * (double)(Ts * EB) / (fill_round * 1e-3);
     }
+    for (int b = 0; b < kPartBufs; b++) t->pref_chunk[b] = -1;   // buffers reused from here on
     rc.overload_ms += now_ms() - t_begin;
     if (ctx->cfg.verbose > 1)
-        printf("stream round %u: pile %u, %u dense chunks %llu edges, gathered %u vertices %llu edges, "
+        printf("stream round %u at %.3f ms: pile %u, %u dense chunks %llu edges, gathered %u vertices %llu edges, "
                "%.3f ms (prepare %.3f)\n",
-               rc.stream_rounds, npile, n_dense, (unsigned long long)dense_edges, n_sparse,
+               rc.stream_rounds, t_begin - rc.t_start, npile, n_dense, (unsigned long long)dense_edges, n_sparse,
                (unsigned long long)t->h_prefix[n_sparse], now_ms() - t_begin, t_pre - t_begin);
     return ASC_OK;
 }
@@ -996,6 +1031,8 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     t->timed.clear();
     t->ev_next = 0;
     const double t0 = now_ms();
+    rc.t_start = t0;
+    if (g->static_vertices < V && t->part[0]) ASC_TRY(prefetch_chunks(g, t));
     uint32_t level = 0;
     bool last_split_empty = false;
     for (;;) {

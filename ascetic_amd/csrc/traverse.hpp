@@ -15,6 +15,7 @@ constexpr uint32_t kHeavyCap = 1u << 20;        // heavy list entries per level
 constexpr int kTile = 1024;                     // edges per tile in edge-balanced kernels
 constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
 constexpr int kBlocksPerCU = 8;
+constexpr int kPartBufs = 2;                  // device buffers for streamed parts and chunks
 constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
 constexpr uint32_t kMaxChunks = 1024;           // dense on-demand chunks (LDS histogram size)
 
@@ -105,6 +106,7 @@ struct RunCounters {
     double pre_ms = 0, overload_ms = 0, fill_ms = 0;
     uint64_t node_sum = 0, streamed_edges = 0, dense_edges = 0;
     uint32_t parts = 0, stream_rounds = 0, splits = 0, dense_chunks = 0;
+    double t_start = 0;             // host clock at the start of the run (per-round log)
 };
 
 struct TravState {
@@ -127,7 +129,8 @@ struct TravState {
     Slot* slots = nullptr;
     Piles* piles = nullptr;
     DevStats* stats = nullptr;
-    void* part[2] = {nullptr, nullptr};
+    void* part[kPartBufs] = {};     // device part buffers, used round-robin
+    int pref_chunk[kPartBufs] = {};  // chunk copied ahead into each part buffer (-1: none)
     uint64_t part_edges = 0;
     HostBuf<Slot> h_slot;
     HostBuf<Piles> h_piles;
@@ -158,7 +161,7 @@ struct TravState {
     HostBuf<uint32_t> h_sample;
     // device part buffer b: copy into it done / last kernel reading it done;
     // staging buffer i: copy out of it done
-    hipEvent_t ev_copy[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr}, ev_meta = nullptr;
+    hipEvent_t ev_copy[kPartBufs] = {}, ev_kern[kPartBufs] = {}, ev_meta = nullptr;
     hipEvent_t ev_stage[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     // event pairs recorded during the current run: (kernel kind, start, end)
