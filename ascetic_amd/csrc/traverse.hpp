@@ -15,7 +15,7 @@ constexpr uint32_t kHeavyCap = 1u << 20;        // heavy list entries per level
 constexpr int kTile = 1024;                     // edges per tile in edge-balanced kernels
 constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
 constexpr int kBlocksPerCU = 8;
-constexpr int kPartBufs = 2;                  // device buffers for streamed parts and chunks
+constexpr int kPartBufs = 4;                  // device buffers for streamed parts and chunks (round-robin)
 constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
 constexpr uint32_t kMaxChunks = 1024;           // dense on-demand chunks (LDS histogram size)
 

@@ -366,8 +366,8 @@ def stream_section(A, _abi, args):
             "peak_h2d_GBps": round(link_h2d, 2),
             "peak_d2h_GBps": round(link_d2h, 2),
             "frac": round(streamed / step_s / 1e9 / link_h2d, 4) if link_h2d else None,
-            "stream_record_bytes": 4 if int(streamed) == 4 * int(sum(r["streamed_edges"] for r in runs) / args.steps)
-            else 8,
+            "stream_record_bytes": round(sum(r["streamed_bytes"] for r in runs) /
+                                         max(1, sum(r["streamed_edges"] for r in runs))),
             "gather_ms_per_step": sum(r["fill_ms"] for r in runs) / args.steps,
             "prepare_ms_per_step": sum(r["pre_ms"] for r in runs) / args.steps,
             "overload_ms_per_step": sum(r["overload_ms"] for r in runs) / args.steps,

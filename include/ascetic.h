@@ -119,8 +119,9 @@ typedef struct {
     uint64_t edge_budget_bytes;  /* > 0: static region limited to this many
                                     bytes of edge records (overrides
                                     advise_rate); 0: derived from free memory */
-    uint64_t part_bytes;         /* bytes of each of the two on-demand part
-                                    buffers; 0 = 64 MiB                       */
+    uint64_t part_bytes;         /* bytes of each on-demand part buffer (four
+                                    on the device, two pinned staging buffers
+                                    on the host); 0 = 64 MiB                  */
     uint64_t device_mem_limit;   /* > 0: pretend the device has this much free
                                     memory when sizing (testing); 0 = query   */
     int gather_threads;          /* host gather threads; 0 = min(16, cores)   */
