@@ -344,9 +344,9 @@ constexpr uint32_t kD16Inf = 0xFFFFu;   // unreached in a 16-bit distance lane
 // lanes of one vertex share its distance row).  Rows are `stride` (a multiple
 // of 8) u16 long and 16-byte aligned; phase 1 reads both rows in 16-byte
 // groups of 8 lanes with no atomics in between, so the loads of all active
-// groups are in flight together; phase 2 lowers, by compare-and-swap on the
-// 32-bit word holding the lane, only lanes whose candidate beats the value
-// read.  Returns true when v must be queued.
+// groups are in flight together; phase 2 lowers the lanes whose candidate
+// beat the value read, one compare-and-swap per 64-bit word of four lanes.
+// Returns true when v must be queued.
 template <class OffT>
 __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
                                            unsigned long long bits, unsigned long long& act) {
@@ -374,21 +374,33 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
     }
     if (over) atomicOr(a.ovf, over);
     if (!maybe) return false;
-    const uint16_t* dus = a.dist + (size_t)u * a.stride;
-    uint32_t* row = (uint32_t*)(a.dist + (size_t)v * a.stride);
+    // phase 2, per 64-bit word of four lanes: one compare-and-swap lowers
+    // every lane of the word whose candidate beats it
+    const unsigned long long* du64 = (const unsigned long long*)(a.dist + (size_t)u * a.stride);
+    unsigned long long* row = (unsigned long long*)(a.dist + (size_t)v * a.stride);
     unsigned long long imp = 0;
-    for (unsigned long long m = maybe; m; m &= m - 1) {
-        const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
-        const uint32_t c = (uint32_t)dus[q] + wgt;
-        uint32_t* word = row + (q >> 1);
-        const uint32_t sh = 16 * (q & 1);
-        uint32_t old = *word;
+    for (unsigned long long m = maybe; m;) {
+        const uint32_t wi = ((uint32_t)__ffsll((long long)m) - 1) >> 2;
+        const uint32_t nib = (uint32_t)(m >> (4 * wi)) & 0xFu;
+        m &= ~(0xFull << (4 * wi));
+        const unsigned long long xu = du64[wi];
+        unsigned long long old = row[wi];
         for (;;) {
-            if (((old >> sh) & 0xFFFF) <= c) break;
-            const uint32_t nw = (old & ~(0xFFFFu << sh)) | (c << sh);
-            const uint32_t prev = atomicCAS(word, old, nw);
+            unsigned long long nw = old;
+            uint32_t got = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 4; j++) {
+                if (!((nib >> j) & 1u)) continue;
+                const uint32_t c = (uint32_t)((xu >> (16 * j)) & 0xFFFFull) + wgt;
+                if (c < (uint32_t)((old >> (16 * j)) & 0xFFFFull)) {
+                    nw = (nw & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
+                    got |= 1u << j;
+                }
+            }
+            if (!got) break;
+            const unsigned long long prev = atomicCAS(row + wi, old, nw);
             if (prev == old) {
-                imp |= 1ull << q;
+                imp |= (unsigned long long)got << (4 * wi);
                 break;
             }
             old = prev;

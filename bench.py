@@ -52,7 +52,7 @@ def algorithmic_bytes(kern, s_e, s_o):
     }
 
 
-def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None):
+def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None, step_graphs=True):
     """Query serving on a resident weighted R-MAT graph: 70% BFS / 30% SSSP,
     sources uniform over vertices with degree > 0.  Reports one-query-per-
     launch throughput (plain kernels), the server's saturated throughput and
@@ -84,7 +84,7 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None
         ctx.synchronize()
         plain_qps = n_plain / (time.perf_counter() - t0)
         out["plain_qps"] = round(plain_qps, 2)
-        with A.Server(g, lanes=64) as srv:
+        with A.Server(g, lanes=64, step_graphs=step_graphs) as srv:
             st = srv.stats()
             out["lanes"] = {"bfs": st["lanes_bfs"], "sssp": st["lanes_sssp"]}
             # exactness on a sample, served in a full mix
@@ -444,6 +444,8 @@ def main():
     ap.add_argument("--algo-er", type=str, default="26,30",
                     help="log2 V,log2 undirected pairs of the CC/PageRank graph ('' = skip)")
     ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--serve-eager", action="store_true",
+                    help="serving steps launch their kernels one by one instead of replaying a hipGraph")
     ap.add_argument("--resident-scale", type=int, default=26,
                     help="R-MAT scale of the resident BFS/SSSP/CC section (0 = skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
@@ -485,7 +487,8 @@ def main():
     if args.serve_scale > 0:
         # the serving section adopts (consumes) the resident graph when the scales match
         result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0],
-                                          h=hr)
+                                          h=hr, step_graphs=not args.serve_eager)
+        result["serving"]["step_graphs"] = not args.serve_eager
     print(json.dumps(result), flush=True)
     if result.get("parity", "").startswith("MISMATCH"):
         sys.exit(1)
