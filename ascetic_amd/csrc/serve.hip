@@ -353,57 +353,103 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
     const uint4* du4 = (const uint4*)(a.dist + (size_t)u * a.stride);
     const uint4* dv4 = (const uint4*)(a.dist + (size_t)v * a.stride);
     unsigned long long maybe = 0, over = 0;
-    for (unsigned long long gm = bits; gm;) {
-        const uint32_t g = ((uint32_t)__ffsll((long long)gm) - 1) >> 3;
-        gm &= ~(0xFFull << (8 * g));
-        const uint4 x = du4[g], y = dv4[g];
-        const uint32_t xs[4] = {x.x, x.y, x.z, x.w}, ys[4] = {y.x, y.y, y.z, y.w};
-        const uint32_t byte = (uint32_t)(bits >> (8 * g)) & 0xFF;
+    // phase 1, four groups of eight lanes at a time: the loads of a batch's
+    // active groups are issued before any of them is used
+    for (uint32_t gb = 0; gb < kMaxLanes / 8; gb += 4) {
+        const uint32_t bb = (uint32_t)(bits >> (8 * gb));   // lanes of groups gb..gb+3
+        if (!bb) continue;
+        uint4 x[4], y[4];
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            if (!((byte >> j) & 1)) continue;
-            const uint32_t du = (xs[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-            const uint32_t dv = (ys[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-            const uint32_t c = wgt >= kD16Inf ? kD16Inf : du + wgt;   // no 32-bit wrap
-            if (c >= kD16Inf) {
-                over |= 1ull << (8 * g + j);
-            } else if (c < dv) {
-                maybe |= 1ull << (8 * g + j);
+        for (uint32_t k = 0; k < 4; k++)
+            if ((bb >> (8 * k)) & 0xFFu) {
+                x[k] = du4[gb + k];
+                y[k] = dv4[gb + k];
+            }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t byte = (bb >> (8 * k)) & 0xFFu;
+            if (!byte) continue;
+            const uint32_t xs[4] = {x[k].x, x[k].y, x[k].z, x[k].w}, ys[4] = {y[k].x, y[k].y, y[k].z, y[k].w};
+#pragma unroll
+            for (uint32_t j = 0; j < 8; j++) {
+                if (!((byte >> j) & 1u)) continue;
+                const uint32_t du = (xs[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+                const uint32_t dv = (ys[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
+                const uint32_t c = wgt >= kD16Inf ? kD16Inf : du + wgt;   // no 32-bit wrap
+                const unsigned long long bit = 1ull << (8 * (gb + k) + j);
+                if (c >= kD16Inf) over |= bit;
+                else if (c < dv) maybe |= bit;
             }
         }
     }
     if (over) atomicOr(a.ovf, over);
     if (!maybe) return false;
     // phase 2, per 64-bit word of four lanes: one compare-and-swap lowers
-    // every lane of the word whose candidate beats it
+    // every lane of the word whose candidate beats it; the compare-and-swaps
+    // of four words are in flight together, a lost race is retried alone
     const unsigned long long* du64 = (const unsigned long long*)(a.dist + (size_t)u * a.stride);
     unsigned long long* row = (unsigned long long*)(a.dist + (size_t)v * a.stride);
     unsigned long long imp = 0;
-    for (unsigned long long m = maybe; m;) {
-        const uint32_t wi = ((uint32_t)__ffsll((long long)m) - 1) >> 2;
-        const uint32_t nib = (uint32_t)(m >> (4 * wi)) & 0xFu;
-        m &= ~(0xFull << (4 * wi));
-        const unsigned long long xu = du64[wi];
-        unsigned long long old = row[wi];
-        for (;;) {
-            unsigned long long nw = old;
-            uint32_t got = 0;
+    for (uint32_t wb = 0; wb < kMaxLanes / 4; wb += 4) {
+        const uint32_t mb = (uint32_t)(maybe >> (4 * wb)) & 0xFFFFu;   // lanes of words wb..wb+3
+        if (!mb) continue;
+        unsigned long long xu[4], old[4], nw[4], prev[4];
+        uint32_t got[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if ((mb >> (4 * k)) & 0xFu) {
+                xu[k] = du64[wb + k];
+                old[k] = row[wb + k];
+            }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            const uint32_t nib = (mb >> (4 * k)) & 0xFu;
+            got[k] = 0;
+            if (!nib) continue;
+            nw[k] = old[k];
 #pragma unroll
             for (uint32_t j = 0; j < 4; j++) {
                 if (!((nib >> j) & 1u)) continue;
-                const uint32_t c = (uint32_t)((xu >> (16 * j)) & 0xFFFFull) + wgt;
-                if (c < (uint32_t)((old >> (16 * j)) & 0xFFFFull)) {
-                    nw = (nw & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
-                    got |= 1u << j;
+                const uint32_t c = (uint32_t)((xu[k] >> (16 * j)) & 0xFFFFull) + wgt;
+                if (c < (uint32_t)((old[k] >> (16 * j)) & 0xFFFFull)) {
+                    nw[k] = (nw[k] & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
+                    got[k] |= 1u << j;
                 }
             }
-            if (!got) break;
-            const unsigned long long prev = atomicCAS(row + wi, old, nw);
-            if (prev == old) {
-                imp |= (unsigned long long)got << (4 * wi);
-                break;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++)
+            if (got[k]) prev[k] = atomicCAS(row + wb + k, old[k], nw[k]);
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            if (!got[k]) continue;
+            if (prev[k] == old[k]) {
+                imp |= (unsigned long long)got[k] << (4 * (wb + k));
+                continue;
             }
-            old = prev;
+            // lost a race: retry this word against the value that won
+            const uint32_t nib = (mb >> (4 * k)) & 0xFu;
+            unsigned long long o = prev[k];
+            for (;;) {
+                unsigned long long n2 = o;
+                uint32_t g2 = 0;
+#pragma unroll
+                for (uint32_t j = 0; j < 4; j++) {
+                    if (!((nib >> j) & 1u)) continue;
+                    const uint32_t c = (uint32_t)((xu[k] >> (16 * j)) & 0xFFFFull) + wgt;
+                    if (c < (uint32_t)((o >> (16 * j)) & 0xFFFFull)) {
+                        n2 = (n2 & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
+                        g2 |= 1u << j;
+                    }
+                }
+                if (!g2) break;
+                const unsigned long long p2 = atomicCAS(row + wb + k, o, n2);
+                if (p2 == o) {
+                    imp |= (unsigned long long)g2 << (4 * (wb + k));
+                    break;
+                }
+                o = p2;
+            }
         }
     }
     if (!imp) return false;
