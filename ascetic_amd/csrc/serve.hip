@@ -339,6 +339,11 @@ __global__ __launch_bounds__(kBlock) void k_lane_counts(const OffT* off, const u
 // -- SSSP lanes -----------------------------------------------------------------
 
 constexpr uint32_t kD16Inf = 0xFFFFu;   // unreached in a 16-bit distance lane
+// Batches of the lane relax: groups of 8 lanes whose row loads are issued
+// together, 4-lane words whose compare-and-swaps are issued together (wider
+// batches cost occupancy: 8 / 16 ran slower at scale 26).
+constexpr uint32_t kRB = 4;
+constexpr uint32_t kRW = 4;
 
 // Relaxes edge (u, v, w) for every lane in `bits` (one thread per edge; the
 // lanes of one vertex share its distance row).  Rows are `stride` (a multiple
@@ -353,21 +358,21 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
     const uint4* du4 = (const uint4*)(a.dist + (size_t)u * a.stride);
     const uint4* dv4 = (const uint4*)(a.dist + (size_t)v * a.stride);
     unsigned long long maybe = 0, over = 0;
-    // phase 1, four groups of eight lanes at a time: the loads of a batch's
+    // phase 1, kRB groups of eight lanes at a time: the loads of a batch's
     // active groups are issued before any of them is used
-    for (uint32_t gb = 0; gb < kMaxLanes / 8; gb += 4) {
-        const uint32_t bb = (uint32_t)(bits >> (8 * gb));   // lanes of groups gb..gb+3
+    for (uint32_t gb = 0; gb < kMaxLanes / 8; gb += kRB) {
+        const unsigned long long bb = kRB == 8 ? bits : (bits >> (8 * gb)) & ((1ull << (8 * kRB)) - 1);
         if (!bb) continue;
-        uint4 x[4], y[4];
+        uint4 x[kRB], y[kRB];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
+        for (uint32_t k = 0; k < kRB; k++)
             if ((bb >> (8 * k)) & 0xFFu) {
                 x[k] = du4[gb + k];
                 y[k] = dv4[gb + k];
             }
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t byte = (bb >> (8 * k)) & 0xFFu;
+        for (uint32_t k = 0; k < kRB; k++) {
+            const uint32_t byte = (uint32_t)(bb >> (8 * k)) & 0xFFu;
             if (!byte) continue;
             const uint32_t xs[4] = {x[k].x, x[k].y, x[k].z, x[k].w}, ys[4] = {y[k].x, y[k].y, y[k].z, y[k].w};
 #pragma unroll
@@ -386,24 +391,24 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
     if (!maybe) return false;
     // phase 2, per 64-bit word of four lanes: one compare-and-swap lowers
     // every lane of the word whose candidate beats it; the compare-and-swaps
-    // of four words are in flight together, a lost race is retried alone
+    // of kRW words are in flight together, a lost race is retried alone
     const unsigned long long* du64 = (const unsigned long long*)(a.dist + (size_t)u * a.stride);
     unsigned long long* row = (unsigned long long*)(a.dist + (size_t)v * a.stride);
     unsigned long long imp = 0;
-    for (uint32_t wb = 0; wb < kMaxLanes / 4; wb += 4) {
-        const uint32_t mb = (uint32_t)(maybe >> (4 * wb)) & 0xFFFFu;   // lanes of words wb..wb+3
+    for (uint32_t wb = 0; wb < kMaxLanes / 4; wb += kRW) {
+        const unsigned long long mb = kRW == 16 ? maybe : (maybe >> (4 * wb)) & ((1ull << (4 * kRW)) - 1);
         if (!mb) continue;
-        unsigned long long xu[4], old[4], nw[4], prev[4];
-        uint32_t got[4];
+        unsigned long long xu[kRW], old[kRW], nw[kRW], prev[kRW];
+        uint32_t got[kRW];
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
+        for (uint32_t k = 0; k < kRW; k++)
             if ((mb >> (4 * k)) & 0xFu) {
                 xu[k] = du64[wb + k];
                 old[k] = row[wb + k];
             }
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
-            const uint32_t nib = (mb >> (4 * k)) & 0xFu;
+        for (uint32_t k = 0; k < kRW; k++) {
+            const uint32_t nib = (uint32_t)(mb >> (4 * k)) & 0xFu;
             got[k] = 0;
             if (!nib) continue;
             nw[k] = old[k];
@@ -418,17 +423,17 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
             }
         }
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++)
+        for (uint32_t k = 0; k < kRW; k++)
             if (got[k]) prev[k] = atomicCAS(row + wb + k, old[k], nw[k]);
 #pragma unroll
-        for (uint32_t k = 0; k < 4; k++) {
+        for (uint32_t k = 0; k < kRW; k++) {
             if (!got[k]) continue;
             if (prev[k] == old[k]) {
                 imp |= (unsigned long long)got[k] << (4 * (wb + k));
                 continue;
             }
             // lost a race: retry this word against the value that won
-            const uint32_t nib = (mb >> (4 * k)) & 0xFu;
+            const uint32_t nib = (uint32_t)(mb >> (4 * k)) & 0xFu;
             unsigned long long o = prev[k];
             for (;;) {
                 unsigned long long n2 = o;
@@ -672,6 +677,7 @@ struct Engine {
     SQuery* lane_q[kMaxLanes] = {};
     unsigned long long busy = 0;
     uint32_t step = 0;
+    uint32_t group_step = 0;          // step at which the current group of lanes was admitted
     bool launched = false;
     // BFS direction per lane: bottom-up lanes, next frontier vertices and
     // edges (from k_lane_counts), unexplored edges
@@ -989,11 +995,17 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
 // union frontier then spans most of the graph each step).  So while an engine
 // is busy, freed BFS lanes are refilled only as a group of at least half the
 // lanes, and SSSP lanes (whose peaks last longer) only once every lane is
-// free; an idle engine admits at once.  Measured on R-MAT scale 26 at
+// free; an idle engine admits at once, and queries arriving within the
+// first kJoinSteps steps of a group join it (they are still nearly in phase;
+// without this a query that finds the engine idle runs alone while the
+// queries submitted right after it wait for it: 84 vs 90 SSSP queries/s).  Measured on R-MAT scale 26 at
 // saturation against admitting at every step boundary: BFS 979 vs 795
 // queries/s, SSSP 59 vs 48.
+constexpr uint32_t kJoinSteps = 2;
+
 static bool admit_now(const asc_server* s, const Engine& e) {
     if (!e.busy || s->pending[e.kind].empty()) return true;
+    if (e.step - e.group_step < kJoinSteps) return true;   // the group is still in its first steps
     const int free_lanes = e.lanes - __builtin_popcountll(e.busy);
     if (e.kind == ASC_QUERY_SSSP) return false;
     return free_lanes >= std::max(1, e.lanes / 2);
@@ -1017,6 +1029,7 @@ static void server_loop(asc_server* s) {
                 Engine& e = s->eng[k];
                 admitted[k].clear();
                 if (!admit_now(s, e)) continue;
+                if (!e.busy && !s->pending[k].empty()) e.group_step = e.step;   // a new group starts
                 for (int q = 0; q < e.lanes && !s->pending[k].empty(); q++) {
                     if (e.busy & (1ull << q)) continue;
                     SQuery* qq = s->pending[k].front();
