@@ -55,6 +55,61 @@ __device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v,
 // then their probes, are in flight together.
 constexpr int kUnroll = 4;
 
+// relax_commit for a lane's kUnroll edges at once (edge j counts when bit j
+// of `valid` is set): the value atomics of all of them are issued before any
+// result is used, then the stamp atomics of those that improved, so each
+// lane waits for two atomic round trips per pass instead of two per edge.
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ void relax_commit_batch(const TravArgs<OffT, EdgeT>& a, const uint32_t (&v)[kUnroll],
+                                                   const uint32_t (&cand)[kUnroll], const uint32_t (&probe)[kUnroll],
+                                                   uint32_t valid, uint32_t level, int (&to)[kUnroll]) {
+    uint32_t old[kUnroll];
+    bool try_[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++) {
+        to[j] = TO_NONE;
+        if (ALGO == ALG_BFS) try_[j] = ((valid >> j) & 1u) && !(probe[j] & (1u << (v[j] & 31)));
+        else try_[j] = ((valid >> j) & 1u) && cand[j] < probe[j];
+        if (!try_[j]) continue;
+        if (ALGO == ALG_BFS) old[j] = atomicOr(a.visited + (v[j] >> 5), 1u << (v[j] & 31));
+        else old[j] = atomicMin(a.val + v[j], cand[j]);
+    }
+    if (ALGO == ALG_BFS) {
+#pragma unroll
+        for (int j = 0; j < kUnroll; j++) {
+            if (!try_[j] || (old[j] & (1u << (v[j] & 31)))) continue;
+            a.val[v[j]] = cand[j];
+            to[j] = v[j] < a.vs ? TO_RES : TO_PEND;
+        }
+        return;
+    }
+    uint32_t* sp[kUnroll];
+    uint32_t sv[kUnroll], prev[kUnroll];
+    int kind[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++) {
+        kind[j] = TO_NONE;
+        if (!try_[j] || old[j] <= cand[j]) continue;
+        if (ALGO == ALG_SSSP && cand[j] >= a.theta) {
+            sp[j] = a.fstamp + v[j];
+            sv[j] = a.fepoch;
+            kind[j] = TO_FAR;
+        } else if (v[j] < a.vs) {
+            sp[j] = a.stamp + v[j];
+            sv[j] = level + 1;
+            kind[j] = TO_RES;
+        } else {
+            sp[j] = a.pstamp + v[j];
+            sv[j] = a.pepoch;
+            kind[j] = TO_PEND;
+        }
+        prev[j] = atomicMax(sp[j], sv[j]);
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++)
+        if (kind[j] != TO_NONE && prev[j] < sv[j]) to[j] = kind[j];
+}
+
 template <int ALGO>
 struct Pushes {
     WaveStage<kStageCap> res, pend, far;
@@ -149,12 +204,14 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
                 cc[j] = candidate<ALGO>(s_sv[kk[j]], edge_w(ed[j]));
                 pr[j] = relax_probe<ALGO>(a, vv[j]);
             }
+            uint32_t valid = 0;
 #pragma unroll
-            for (int j = 0; j < kUnroll; j++) {
-                int to = TO_NONE;
-                if (eb + j * kWave + lane < total) to = relax_commit<ALGO>(a, vv[j], cc[j], level, pr[j]);
-                pq.push(a, nxt, to, vv[j]);
-            }
+            for (int j = 0; j < kUnroll; j++)
+                if (eb + j * kWave + lane < total) valid |= 1u << j;
+            int to[kUnroll];
+            relax_commit_batch<ALGO>(a, vv, cc, pr, valid, level, to);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) pq.push(a, nxt, to[j], vv[j]);
         }
         my_edges += total;
         my_verts += cnt;
@@ -242,12 +299,14 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
                 cc[j] = candidate<ALGO>(s_hsv[kk[j]], edge_w(ed[j]));
                 pr[j] = relax_probe<ALGO>(a, vv[j]);
             }
+            uint32_t valid = 0;
 #pragma unroll
-            for (int j = 0; j < kUnroll; j++) {
-                int to = TO_NONE;
-                if (r + j * kWave + lane < tlen) to = relax_commit<ALGO>(a, vv[j], cc[j], level, pr[j]);
-                pq.push(a, nxt, to, vv[j]);
-            }
+            for (int j = 0; j < kUnroll; j++)
+                if (r + j * kWave + lane < tlen) valid |= 1u << j;
+            int to[kUnroll];
+            relax_commit_batch<ALGO>(a, vv, cc, pr, valid, level, to);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) pq.push(a, nxt, to[j], vv[j]);
         }
         my_edges += tlen;
         __syncthreads();

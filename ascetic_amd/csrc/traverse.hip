@@ -145,15 +145,17 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
                 }
                 kk[j] = k;
             }
-#pragma unroll
-            for (int j = 0; j < kUnroll; j++) pr[j] = relax_probe<ALGO>(a, vv[j]);
+            uint32_t cc[kUnroll], valid = 0;
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) {
-                int to = TO_NONE;
-                if (r + j * kWave + lane < tlen)
-                    to = relax_commit<ALGO>(a, vv[j], candidate<ALGO>(s_sv[kk[j]], ww[j]), level, pr[j]);
-                pq.push(a, nxt, to, vv[j]);
+                pr[j] = relax_probe<ALGO>(a, vv[j]);
+                cc[j] = candidate<ALGO>(s_sv[kk[j]], ww[j]);
+                if (r + j * kWave + lane < tlen) valid |= 1u << j;
             }
+            int to[kUnroll];
+            relax_commit_batch<ALGO>(a, vv, cc, pr, valid, level, to);
+#pragma unroll
+            for (int j = 0; j < kUnroll; j++) pq.push(a, nxt, to[j], vv[j]);
         }
         my_edges += tlen;
         my_verts += m;
