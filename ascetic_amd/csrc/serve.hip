@@ -232,6 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a
     if (lane == 0 && act) atomicOr(&nxt->pad[0], act);
 }
 
+// Neighbour ids loaded before their frontier words are probed (8 and 16 ran
+// slower: the scan of a vertex usually ends within its first few neighbours).
+constexpr int kBuBatch = 4;
+
 // One bottom-up step for the lanes in the step's bottom-up mask (symmetric
 // graph: in-neighbours are the adjacency); returns at once when it is empty.
 // Thread per vertex; four neighbour ids are loaded before their frontier
@@ -253,12 +257,12 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, Edge
             if (need) {
                 const OffT b = a.off[v], e = a.off[v + 1];
                 unsigned long long nw = 0;
-                for (OffT i = b; i < e && nw != need; i += 4) {
-                    uint32_t u[4];
+                for (OffT i = b; i < e && nw != need; i += kBuBatch) {
+                    uint32_t u[kBuBatch];
 #pragma unroll
-                    for (int j = 0; j < 4; j++) u[j] = edge_dst(a.edges[min(i + (OffT)j, e - 1)]);
+                    for (int j = 0; j < kBuBatch; j++) u[j] = edge_dst(a.edges[min(i + (OffT)j, e - 1)]);
 #pragma unroll
-                    for (int j = 0; j < 4; j++) nw |= a.X[u[j]] & need;
+                    for (int j = 0; j < kBuBatch; j++) nw |= a.X[u[j]] & need;
                 }
                 if (nw) {
                     const unsigned long long old = atomicOr(a.seen + v, nw);
