@@ -285,6 +285,23 @@ __global__ __launch_bounds__(kBlock) void k_do_fill(uint32_t* p, uint64_t n, uin
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) p[i] = v;
 }
 
+// End of a level: the level's counters (and the next top-down frontier's
+// size, in `pad`) are written to mapped host memory, and the device counters
+// are cleared for the next level (no copy or memset dispatch per level).
+__global__ void k_do_publish(DoCounters* c, const Slot* next_slot, DoCounters* host) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const DoCounters v = *c;
+    host->n_found = v.n_found;
+    host->m_found = v.m_found;
+    host->m_frontier = v.m_frontier;
+    host->pad = next_slot ? next_slot->n_res : 0;
+    __threadfence_system();
+    c->n_found = 0;
+    c->m_found = 0;
+    c->m_frontier = 0;
+    c->pad = 0;
+}
+
 // Level-0 frontier: the source alone, value 1 (level + 1).
 __global__ void k_do_source(uint32_t* val, uint32_t* visited, uint32_t* q, Slot* slot, uint32_t src) {
     val[src] = 1;
@@ -304,6 +321,7 @@ static int ensure_fast_state(asc_graph* g) {
     if (!t->dcnt) {
         ASC_HIP(hipMalloc(&t->dcnt, sizeof(DoCounters)));
         ASC_TRY(t->h_dcnt.alloc(1, true));
+        ASC_HIP(hipHostGetDevicePointer((void**)&t->d_hdcnt, t->h_dcnt.p, 0));
         ASC_TRY(t->h_sample.alloc(kSamples, true));
     }
     return ASC_OK;
@@ -323,8 +341,9 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
     const int grid = ctx->num_cus * (g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
     const int bu_grid = (int)std::max<uint64_t>(1, std::min<uint64_t>((V + kBlock - 1) / kBlock, (uint64_t)grid));
 
-    ASC_HIP(hipMemsetAsync(t->slots, 0, sizeof(Slot) * 4, s));
-    ASC_HIP(hipMemsetAsync(t->stats, 0, sizeof(DevStats), s));
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->slots, sizeof(Slot) * 4 / 4, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->stats, sizeof(DevStats) / 4, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->dcnt, sizeof(DoCounters) / 4, 0u);
     hipLaunchKernelGGL(k_do_fill, dim3(fill_grid), dim3(kBlock), 0, s, t->val, V, (uint32_t)(V + 1));
     hipLaunchKernelGGL(k_do_fill, dim3(fill_grid), dim3(kBlock), 0, s, t->visited, words, 0u);
     hipLaunchKernelGGL(k_do_source, dim3(1), dim3(1), 0, s, t->val, t->visited, t->q[0], t->slots, src);
@@ -370,13 +389,12 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             bottom_up = true;
         } else if (bottom_up && nf < V / kBeta && nf < prev_nf) {
             // bitmap -> queue, slot ring restarted at this level
-            ASC_HIP(hipMemsetAsync(t->slots, 0, sizeof(Slot) * 4, s));
+            hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->slots, sizeof(Slot) * 4 / 4, 0u);
             hipLaunchKernelGGL(k_bitmap_to_queue, dim3(std::max<uint64_t>(1, std::min<uint64_t>((words + kBlock - 1) / kBlock, grid))),
                                dim3(kBlock), 0, s, t->bm[cur], (uint32_t)V,
                                t->q[level & 1], &t->slots[level & 3].n_res);
             bottom_up = false;
         }
-        ASC_HIP(hipMemsetAsync(t->dcnt, 0, sizeof(DoCounters), s));
         if (bottom_up) {
             ASC_TRY(timed_launch(t, prof, KERN_BU, s, [&] {
                 hipLaunchKernelGGL((k_bfs_bottom_up<OffT, EdgeT>), dim3(bu_grid), dim3(kBlock), 0, s, a.off, a.edges,
@@ -398,7 +416,6 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             hipLaunchKernelGGL((k_frontier_edges<OffT>), dim3(grid), dim3(kBlock), 0, s, a.off, a.q_out,
                                t->slots + ((level + 1) & 3), &t->dcnt->m_frontier);
             ASC_HIP(hipGetLastError());
-            ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + ((level + 1) & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
         }
         if (!prof) {
             hipEvent_t e1 = next_event(t);
@@ -406,15 +423,18 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             ASC_HIP(hipEventRecord(e1, s));
             t->timed.push_back({-1, e0, e1});
         }
-        ASC_HIP(hipMemcpyAsync(t->h_dcnt.p, t->dcnt, sizeof(DoCounters), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_do_publish, dim3(1), dim3(64), 0, s, t->dcnt,
+                           bottom_up ? (const Slot*)nullptr : t->slots + ((level + 1) & 3), t->d_hdcnt);
+        ASC_HIP(hipGetLastError());
         ASC_HIP(hipStreamSynchronize(s));
         prev_nf = nf;
+        const volatile DoCounters* pub = t->h_dcnt.p;   // written by k_do_publish
         if (bottom_up) {
-            nf = t->h_dcnt[0].n_found;
-            mf = t->h_dcnt[0].m_found;
+            nf = pub->n_found;
+            mf = pub->m_found;
         } else {
-            nf = t->h_slot[0].n_res;
-            mf = t->h_dcnt[0].m_frontier;
+            nf = pub->pad;
+            mf = pub->m_frontier;
         }
         m_unexplored = m_unexplored > mf ? m_unexplored - mf : 0;
         if (ctx->cfg.verbose > 1)

@@ -157,7 +157,8 @@ struct TravState {
     // resident fast paths: frontier bitmaps, counters, sampled labels
     uint32_t* bm[2] = {nullptr, nullptr};
     DoCounters* dcnt = nullptr;
-    HostBuf<DoCounters> h_dcnt;
+    HostBuf<DoCounters> h_dcnt;     // written by the device through d_hdcnt (mapped)
+    DoCounters* d_hdcnt = nullptr;
     HostBuf<uint32_t> h_sample;
     // device part buffer b: copy into it done / last kernel reading it done;
     // staging buffer i: copy out of it done
