@@ -349,6 +349,36 @@ constexpr uint32_t kD16Inf = 0xFFFFu;   // unreached in a 16-bit distance lane
 constexpr uint32_t kRB = 4;
 constexpr uint32_t kRW = 4;
 
+// Packed 16-bit lane arithmetic (v_pk_add_u16 with clamp, v_pk_min_u16): two
+// distance lanes per 32-bit register.
+typedef unsigned short u16x2_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_add_sat16(uint32_t x, uint32_t y) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_add_sat(__builtin_bit_cast(u16x2_t, x),
+                                                                      __builtin_bit_cast(u16x2_t, y)));
+}
+__device__ __forceinline__ uint32_t pk_min16(uint32_t x, uint32_t y) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2_t, x),
+                                                                  __builtin_bit_cast(u16x2_t, y)));
+}
+__device__ __forceinline__ unsigned long long pk_min16x4(unsigned long long x, unsigned long long y) {
+    return (unsigned long long)pk_min16((uint32_t)x, (uint32_t)y) |
+           ((unsigned long long)pk_min16((uint32_t)(x >> 32), (uint32_t)(y >> 32)) << 32);
+}
+// du + w (saturating) for the lanes of a 4-lane word set in `nib`, 0xFFFF in the others.
+__device__ __forceinline__ unsigned long long word_candidates(unsigned long long du, uint32_t w2, uint32_t nib) {
+    const uint32_t mlo = ((nib & 1u) ? 0xFFFFu : 0u) | ((nib & 2u) ? 0xFFFF0000u : 0u);
+    const uint32_t mhi = ((nib & 4u) ? 0xFFFFu : 0u) | ((nib & 8u) ? 0xFFFF0000u : 0u);
+    const uint32_t clo = pk_add_sat16((uint32_t)du, w2) | ~mlo;
+    const uint32_t chi = pk_add_sat16((uint32_t)(du >> 32), w2) | ~mhi;
+    return (unsigned long long)clo | ((unsigned long long)chi << 32);
+}
+// Lanes (bit j for lane j of the word) whose 16-bit values differ.
+__device__ __forceinline__ uint32_t lanes_changed(unsigned long long x, unsigned long long y) {
+    const unsigned long long d = x ^ y;
+    return ((d & 0xFFFFull) ? 1u : 0u) | ((d & 0xFFFF0000ull) ? 2u : 0u) | ((d & 0xFFFF00000000ull) ? 4u : 0u) |
+           ((d & 0xFFFF000000000000ull) ? 8u : 0u);
+}
+
 // Relaxes edge (u, v, w) for every lane in `bits` (one thread per edge; the
 // lanes of one vertex share its distance row).  Rows are `stride` (a multiple
 // of 8) u16 long and 16-byte aligned; phase 1 reads both rows in 16-byte
@@ -361,6 +391,7 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
                                            unsigned long long bits, unsigned long long& act) {
     const uint4* du4 = (const uint4*)(a.dist + (size_t)u * a.stride);
     const uint4* dv4 = (const uint4*)(a.dist + (size_t)v * a.stride);
+    const uint32_t w2 = (wgt >= kD16Inf ? kD16Inf : wgt) * 0x10001u;   // the weight in both halves
     unsigned long long maybe = 0, over = 0;
     // phase 1, kRB groups of eight lanes at a time: the loads of a batch's
     // active groups are issued before any of them is used
@@ -379,16 +410,20 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
             const uint32_t byte = (uint32_t)(bb >> (8 * k)) & 0xFFu;
             if (!byte) continue;
             const uint32_t xs[4] = {x[k].x, x[k].y, x[k].z, x[k].w}, ys[4] = {y[k].x, y[k].y, y[k].z, y[k].w};
+            // two lanes per 32-bit word: candidate = du + w saturating at
+            // 0xFFFF (overflow), improved where min(candidate, dv) != dv
+            uint32_t imp8 = 0, ov8 = 0;
 #pragma unroll
-            for (uint32_t j = 0; j < 8; j++) {
-                if (!((byte >> j) & 1u)) continue;
-                const uint32_t du = (xs[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-                const uint32_t dv = (ys[j >> 1] >> (16 * (j & 1))) & 0xFFFF;
-                const uint32_t c = wgt >= kD16Inf ? kD16Inf : du + wgt;   // no 32-bit wrap
-                const unsigned long long bit = 1ull << (8 * (gb + k) + j);
-                if (c >= kD16Inf) over |= bit;
-                else if (c < dv) maybe |= bit;
+            for (uint32_t q = 0; q < 4; q++) {
+                const uint32_t c = pk_add_sat16(xs[q], w2);
+                const uint32_t d = pk_min16(c, ys[q]) ^ ys[q];
+                imp8 |= ((d & 0xFFFFu) != 0 ? 1u : 0u) << (2 * q);
+                imp8 |= ((d >> 16) != 0 ? 1u : 0u) << (2 * q + 1);
+                ov8 |= ((c & 0xFFFFu) == 0xFFFFu ? 1u : 0u) << (2 * q);
+                ov8 |= ((c >> 16) == 0xFFFFu ? 1u : 0u) << (2 * q + 1);
             }
+            maybe |= (unsigned long long)(imp8 & byte) << (8 * (gb + k));
+            over |= (unsigned long long)(ov8 & byte) << (8 * (gb + k));
         }
     }
     if (over) atomicOr(a.ovf, over);
@@ -410,21 +445,16 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
                 xu[k] = du64[wb + k];
                 old[k] = row[wb + k];
             }
+        unsigned long long cand[kRW];
 #pragma unroll
         for (uint32_t k = 0; k < kRW; k++) {
             const uint32_t nib = (uint32_t)(mb >> (4 * k)) & 0xFu;
             got[k] = 0;
             if (!nib) continue;
-            nw[k] = old[k];
-#pragma unroll
-            for (uint32_t j = 0; j < 4; j++) {
-                if (!((nib >> j) & 1u)) continue;
-                const uint32_t c = (uint32_t)((xu[k] >> (16 * j)) & 0xFFFFull) + wgt;
-                if (c < (uint32_t)((old[k] >> (16 * j)) & 0xFFFFull)) {
-                    nw[k] = (nw[k] & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
-                    got[k] |= 1u << j;
-                }
-            }
+            // candidates of the word's lanes in `nib`, 0xFFFF (no change) elsewhere
+            cand[k] = word_candidates(xu[k], w2, nib);
+            nw[k] = pk_min16x4(cand[k], old[k]);
+            got[k] = lanes_changed(nw[k], old[k]);
         }
 #pragma unroll
         for (uint32_t k = 0; k < kRW; k++)
@@ -437,20 +467,10 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
                 continue;
             }
             // lost a race: retry this word against the value that won
-            const uint32_t nib = (uint32_t)(mb >> (4 * k)) & 0xFu;
             unsigned long long o = prev[k];
             for (;;) {
-                unsigned long long n2 = o;
-                uint32_t g2 = 0;
-#pragma unroll
-                for (uint32_t j = 0; j < 4; j++) {
-                    if (!((nib >> j) & 1u)) continue;
-                    const uint32_t c = (uint32_t)((xu[k] >> (16 * j)) & 0xFFFFull) + wgt;
-                    if (c < (uint32_t)((o >> (16 * j)) & 0xFFFFull)) {
-                        n2 = (n2 & ~(0xFFFFull << (16 * j))) | ((unsigned long long)c << (16 * j));
-                        g2 |= 1u << j;
-                    }
-                }
+                const unsigned long long n2 = pk_min16x4(cand[k], o);
+                const uint32_t g2 = lanes_changed(n2, o);
                 if (!g2) break;
                 const unsigned long long p2 = atomicCAS(row + wb + k, o, n2);
                 if (p2 == o) {
