@@ -6,8 +6,11 @@
 //   BFS   bitset-per-vertex multi-source BFS: seen[v] and the frontier words
 //         X[v] / Y[v] are 64-bit, bit q = lane q, so one scan of an edge
 //         (u, v) advances every lane whose frontier holds u:
-//         new = X[u] & ~seen[v].  Levels are kept lane-major (lev[q][v]), so
-//         a finished query's result is one contiguous copy.
+//         new = X[u] & ~seen[v].  Levels are kept vertex-major in 8 bits,
+//         lev[v][q] = hops + 1 up to 254 (a row of 64 lanes is one 64-byte
+//         line, so the lanes that reach a vertex in one step write one line);
+//         a lane whose levels go beyond is answered by the single-query
+//         engine when it leaves its lane.
 //   SSSP  distance lanes: dist[v][q] vertex-major in 16 bits (a row of 64
 //         lanes is one 128-byte line), one thread per edge relaxing every
 //         lane whose frontier holds the edge's source; improved lanes are
@@ -27,7 +30,8 @@
 // as the single-query engine.  A lane whose query produced no new frontier bit
 // in a step is finished; its result is copied out (if requested) and the lane
 // is free for the next waiting query at the next step boundary.  Lanes are
-// granted up to the device-memory budget (4 V bytes of per-query state each).
+// granted up to the device-memory budget (V bytes of per-query state per BFS
+// lane, 2 V per SSSP lane).
 // Every query's values equal those of the single-query engine: BFS levels are
 // level-synchronous per lane, and SSSP lanes converge to the unique distances.
 #include <algorithm>
@@ -66,9 +70,9 @@ struct BatchArgs {
     unsigned long long* X;           // frontier bits of this step (read and cleared)
     unsigned long long* Y;           // frontier bits of the next step
     unsigned long long* seen;        // BFS
-    uint32_t* lev;                   // BFS: lev[q * V + v]
+    uint8_t* lev;                    // BFS: lev[v * stride + q], hops + 1, 0xFF = unreached
     uint16_t* dist;                  // SSSP: dist[v * stride + q], 0xFFFF = unreached
-    unsigned long long* ovf;         // SSSP: lanes whose distances left 16 bits
+    unsigned long long* ovf;         // lanes whose levels / distances left 8 / 16 bits
     const uint32_t* q_in;
     uint32_t* q_out;
     Slot* slots;                     // pad[0] of slot (s+1)%4: lanes that advanced in step s
@@ -84,6 +88,25 @@ struct BatchArgs {
 
 // -- BFS lanes ------------------------------------------------------------------
 
+constexpr uint32_t kLev8Max = 0xFEu;   // largest level an 8-bit BFS lane holds (0xFF: unreached)
+
+// Level of v in the lanes of `nw` (each (v, lane) is set by exactly one thread,
+// the one whose seen update found the lane new): byte stores into v's row; a
+// level beyond 8 bits marks the lane overflowed.
+template <class OffT, class EdgeT>
+__device__ __forceinline__ void msbfs_set_levels(const BatchArgs<OffT, EdgeT>& a, uint32_t v, unsigned long long nw,
+                                                 uint32_t step) {
+    uint8_t* row = a.lev + (size_t)v * a.stride;
+    unsigned long long over = 0;
+    for (unsigned long long m = nw; m; m &= m - 1) {
+        const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
+        const uint32_t l = step - a.s0[q] + 2;
+        if (l > kLev8Max) over |= 1ull << q;
+        row[q] = (uint8_t)(l > kLev8Max ? kLev8Max : l);
+    }
+    if (over) atomicOr(a.ovf, over);
+}
+
 template <class OffT, class EdgeT>
 __device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uint32_t v, unsigned long long bits,
                                             uint32_t step, unsigned long long& act) {
@@ -93,10 +116,7 @@ __device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uin
     nw &= ~old;
     if (!nw) return false;
     act |= nw;
-    for (unsigned long long m = nw; m; m &= m - 1) {
-        const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
-        a.lev[(size_t)q * a.V + v] = step - a.s0[q] + 2;
-    }
+    msbfs_set_levels(a, v, nw, step);
     return atomicOr(a.Y + v, nw) == 0;
 }
 
@@ -269,10 +289,7 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_bottom_up(BatchArgs<OffT, Edge
                     nw &= ~old;
                     if (nw) {
                         act |= nw;
-                        for (unsigned long long m = nw; m; m &= m - 1) {
-                            const uint32_t q = (uint32_t)__ffsll((long long)m) - 1;
-                            a.lev[(size_t)q * a.V + v] = step - a.s0[q] + 2;
-                        }
+                        msbfs_set_levels(a, (uint32_t)v, nw, step);
                         push = atomicOr(a.Y + v, nw) == 0;
                     }
                 }
@@ -629,9 +646,25 @@ __global__ void k_clear_seen_bits(unsigned long long* seen, uint64_t V, unsigned
         seen[v] &= ~mask;
 }
 
-__global__ void k_reset_dist(uint16_t* dist, uint64_t V, uint32_t stride, unsigned long long mask) {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
-        for (unsigned long long m = mask; m; m &= m - 1) dist[v * stride + (__ffsll((long long)m) - 1)] = kD16Inf;
+// Lanes of a state row set to all ones (BFS level 0xFF, SSSP distance
+// 0xFFFF: unreached): `m` has the lanes' bytes set, rows are nu4 16-byte units
+// (at most 8); each thread ORs whole units of one row.
+struct RowMask {
+    uint4 m[8];
+};
+
+__global__ void k_reset_rows(uint4* rows, uint64_t V, uint32_t nu4, RowMask m) {
+    const uint64_t n = V * nu4;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint4 mk = m.m[i % nu4];
+        if (!(mk.x | mk.y | mk.z | mk.w)) continue;
+        uint4 x = rows[i];
+        x.x |= mk.x;
+        x.y |= mk.y;
+        x.z |= mk.z;
+        x.w |= mk.w;
+        rows[i] = x;
+    }
 }
 
 template <class OffT, class EdgeT>
@@ -643,14 +676,22 @@ __global__ void k_seed(BatchArgs<OffT, EdgeT> a, SeedArgs sa, uint32_t step, uin
     s0[q] = step;
     if (bfs) {
         atomicOr(a.seen + src, bit);
-        a.lev[(size_t)q * a.V + src] = 1;
+        a.lev[(size_t)src * a.stride + q] = 1;
     } else {
         a.dist[(size_t)src * a.stride + q] = 1;
-        atomicAnd(a.ovf, ~bit);
     }
+    atomicAnd(a.ovf, ~bit);
     if (atomicOr(a.X + src, bit) == 0) {
         const uint32_t pos = atomicAdd(&a.slots[step & 3].n_res, 1u);
         ((uint32_t*)a.q_in)[pos] = src;
+    }
+}
+
+// Result column of one BFS lane, unreached as V + 1.
+__global__ void k_extract_lev(const uint8_t* lev, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t l = lev[v * stride + lane];
+        out[v] = l == 0xFFu ? (uint32_t)(V + 1) : l;
     }
 }
 
@@ -772,11 +813,11 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     if (kind == ASC_QUERY_BFS) {
         ASC_HIP(hipMalloc(&e.seen, 8 * V));
         ASC_HIP(hipMemsetAsync(e.seen, 0, 8 * V, e.stream));
-    } else {
-        ASC_HIP(hipMalloc(&e.staging, 4 * V));
     }
-    e.stride = kind == ASC_QUERY_SSSP ? (uint32_t)((lanes + 7) / 8 * 8) : (uint32_t)lanes;
-    hipError_t r = hipMalloc(&e.lanes_state, (kind == ASC_QUERY_SSSP ? 2 : 4) * V * (size_t)e.stride);
+    ASC_HIP(hipMalloc(&e.staging, 4 * V));
+    // rows of whole 16-byte units: 16 lanes for the u8 BFS levels, 8 for the u16 SSSP distances
+    e.stride = kind == ASC_QUERY_BFS ? (uint32_t)((lanes + 15) / 16 * 16) : (uint32_t)((lanes + 7) / 8 * 8);
+    hipError_t r = hipMalloc(&e.lanes_state, (kind == ASC_QUERY_SSSP ? 2 : 1) * V * (size_t)e.stride);
     if (r != hipSuccess) {
         set_last_error("serving state of %d lanes x %llu vertices does not fit", lanes, (unsigned long long)V);
         return ASC_ERR_NOMEM;
@@ -790,12 +831,10 @@ static int engine_init(Engine& e, int kind, int lanes, uint64_t V) {
     ASC_TRY(e.h_slot.alloc(4, true));
     ASC_HIP(hipMalloc(&e.d_sp, sizeof(StepParams)));
     ASC_TRY(e.h_sp.alloc(1, true));
-    if (kind == ASC_QUERY_SSSP) {
-        ASC_HIP(hipMalloc(&e.d_ovf, 8));
-        ASC_HIP(hipMemsetAsync(e.d_ovf, 0, 8, e.stream));
-        ASC_TRY(e.h_ovf.alloc(1, true));
-        e.h_ovf[0] = 0;
-    }
+    ASC_HIP(hipMalloc(&e.d_ovf, 8));
+    ASC_HIP(hipMemsetAsync(e.d_ovf, 0, 8, e.stream));
+    ASC_TRY(e.h_ovf.alloc(1, true));
+    e.h_ovf[0] = 0;
     if (kind == ASC_QUERY_BFS) {
         ASC_HIP(hipMalloc(&e.d_cnt, 16 * kMaxLanes));
         ASC_TRY(e.h_cnt.alloc(2 * kMaxLanes, true));
@@ -812,7 +851,7 @@ static BatchArgs<OffT, EdgeT> engine_args(asc_graph* g, Engine& e) {
     a.X = e.bits[e.step & 1];
     a.Y = e.bits[(e.step + 1) & 1];
     a.seen = e.seen;
-    a.lev = e.kind == ASC_QUERY_BFS ? e.lanes_state : nullptr;
+    a.lev = e.kind == ASC_QUERY_BFS ? (uint8_t*)e.lanes_state : nullptr;
     a.dist = e.kind == ASC_QUERY_SSSP ? (uint16_t*)e.lanes_state : nullptr;
     a.ovf = e.d_ovf;
     a.q_in = e.q[e.step & 1];
@@ -898,7 +937,6 @@ static int record_step(asc_graph* g, Engine& e, const BatchArgs<OffT, EdgeT>& a,
 template <class OffT, class EdgeT>
 static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& admitted) {
     const int grid = g->ctx->num_cus * (g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
-    const uint32_t unreached = (uint32_t)(e.V + 1);
     if (e.kind == ASC_QUERY_BFS) choose_directions(g, e, admitted);
     auto a = engine_args<OffT, EdgeT>(g, e);
     if (!admitted.empty()) {
@@ -912,12 +950,19 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
         }
         if (e.kind == ASC_QUERY_BFS) {
             hipLaunchKernelGGL(k_clear_seen_bits, dim3(grid), dim3(256), 0, e.stream, e.seen, e.V, mask);
-            for (SQuery* q : admitted)
-                hipLaunchKernelGGL(k_lane_fill, dim3(grid), dim3(256), 0, e.stream,
-                                   e.lanes_state + (size_t)q->lane * e.V, e.V, unreached);
-        } else {
-            hipLaunchKernelGGL(k_reset_dist, dim3(grid), dim3(256), 0, e.stream, (uint16_t*)e.lanes_state, e.V,
-                               e.stride, mask);
+        }
+        {
+            // the admitted lanes' state: every vertex unreached
+            const uint32_t lane_bytes = e.kind == ASC_QUERY_BFS ? 1 : 2;
+            const uint32_t row_bytes = e.stride * lane_bytes;
+            RowMask rm{};
+            uint8_t* mb = (uint8_t*)rm.m;
+            for (unsigned long long m = mask; m; m &= m - 1) {
+                const uint32_t q = (uint32_t)__builtin_ctzll(m);
+                for (uint32_t b = 0; b < lane_bytes; b++) mb[q * lane_bytes + b] = 0xFF;
+            }
+            hipLaunchKernelGGL(k_reset_rows, dim3(grid), dim3(256), 0, e.stream, (uint4*)e.lanes_state, e.V,
+                               row_bytes / 16, rm);
         }
         hipLaunchKernelGGL((k_seed<OffT, EdgeT>), dim3(1), dim3(kMaxLanes), 0, e.stream, a, sa, e.step, e.s0,
                            e.kind == ASC_QUERY_BFS ? 1 : 0);
@@ -980,22 +1025,21 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
         if (!qq) continue;
         qq->steps++;
         if (act & (1ull << q)) continue;
-        const bool overflowed = e.kind == ASC_QUERY_SSSP && ((e.h_ovf[0] >> q) & 1ull);
+        const bool overflowed = (e.h_ovf[0] >> q) & 1ull;
         if (qq->out && overflowed) {
-            // a distance left 16 bits: the single-query engine answers exactly
-            // (value convention applied there)
-            ASC_TRY(traverse(s->g, ALG_SSSP, qq->src, qq->out, nullptr));
+            // a level left 8 bits / a distance left 16 bits: the single-query
+            // engine answers exactly (value convention applied there)
+            ASC_TRY(traverse(s->g, e.kind == ASC_QUERY_SSSP ? ALG_SSSP : ALG_BFS, qq->src, qq->out, nullptr));
             s->fallbacks++;
         } else if (qq->out) {
-            if (e.kind == ASC_QUERY_BFS) {
-                ASC_HIP(hipMemcpyAsync(qq->out, e.lanes_state + (size_t)q * e.V, 4 * e.V, hipMemcpyDeviceToHost,
-                                       e.stream));
-            } else {
-                const int grid = s->g->ctx->num_cus * 4;
+            const int grid = s->g->ctx->num_cus * 4;
+            if (e.kind == ASC_QUERY_BFS)
+                hipLaunchKernelGGL(k_extract_lev, dim3(grid), dim3(256), 0, e.stream, (const uint8_t*)e.lanes_state,
+                                   e.V, e.stride, (uint32_t)q, e.staging);
+            else
                 hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream,
                                    (const uint16_t*)e.lanes_state, e.V, e.stride, (uint32_t)q, e.staging);
-                ASC_HIP(hipMemcpyAsync(qq->out, e.staging, 4 * e.V, hipMemcpyDeviceToHost, e.stream));
-            }
+            ASC_HIP(hipMemcpyAsync(qq->out, e.staging, 4 * e.V, hipMemcpyDeviceToHost, e.stream));
             ASC_HIP(hipStreamSynchronize(e.stream));
             if (zero_based) {
                 uint32_t* o = qq->out;
@@ -1134,8 +1178,8 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
     uint64_t budget = o.state_budget_bytes ? o.state_budget_bytes : free_b / 2;
     budget = std::min<uint64_t>(budget, free_b > 2 * fixed ? free_b - 2 * fixed : 0);
     const bool weighted = g->h.kind == ASC_KIND_WCSR;
-    // per query: u32 levels (BFS), u16 distances (SSSP)
-    const uint64_t per_lane = weighted ? 6 * V : 4 * V;
+    // per query: u8 levels (BFS), u16 distances (SSSP)
+    const uint64_t per_lane = weighted ? 3 * V : V;
     int lanes = (int)std::min<uint64_t>((uint64_t)o.lanes, budget / per_lane);
     if (lanes < 1) {
         set_last_error("state budget %llu bytes admits no query (%llu bytes per lane)", (unsigned long long)budget,

@@ -406,7 +406,7 @@ typedef enum { ASC_QUERY_BFS = 0, ASC_QUERY_SSSP = 1 } asc_query_kind;
 typedef struct {
     int lanes;                   /* max queries in flight per kind (1..64);
                                     0 = the graph's tuned width (64 untuned) */
-    uint64_t state_budget_bytes; /* device bytes for per-query state (4 V bytes
+    uint64_t state_budget_bytes; /* device bytes for per-query state (V bytes
                                     per BFS lane, 2 V per SSSP lane); 0 = half
                                     the free memory                           */
     int queue_capacity;          /* max submitted-but-unfinished queries      */
@@ -435,8 +435,9 @@ typedef struct {
     int lanes_bfs, lanes_sssp;   /* lanes granted by the memory budget        */
     double busy_ms;              /* wall time with at least one query in flight */
     uint64_t bottom_up_steps;    /* BFS engine steps with bottom-up lanes      */
-    uint64_t fallbacks;          /* SSSP queries whose distances left 16 bits,
-                                    answered by the single-query engine      */
+    uint64_t fallbacks;          /* queries whose BFS levels left 8 bits or SSSP
+                                    distances 16 bits, answered by the
+                                    single-query engine                      */
     uint64_t graph_steps;        /* engine steps replayed from a hipGraph     */
 } asc_server_stats;
 

@@ -154,6 +154,30 @@ def test_sssp_lanes_beyond_16_bits_fall_back_exactly():
             assert np.array_equal(o, O.dijkstra(csr, s_)), s_
 
 
+def test_bfs_lanes_beyond_8_bits_fall_back_exactly():
+    """Levels above 254 do not fit an 8-bit lane: such queries are answered
+    by the single-query engine, and every result stays exact."""
+    from tests import graphs as G
+    n = 600
+    off, dst, _ = G.path_graph(n)
+    off2, dst2, _ = G.random_graph(200, 600, 23, symmetric=True)
+    offs = np.concatenate([off, off2[1:] + off[-1]]).astype(np.uint64)
+    dsts = np.concatenate([dst, dst2 + n]).astype(np.uint32)
+    V = len(offs) - 1
+    csr = O.CSR(offs, dsts)
+    with A.Context() as ctx, ctx.from_arrays(offs, dsts) as g:
+        srcs = [0, 100, n - 1, 300, n, n + 17, n + 150]
+        outs = [np.empty(V, np.uint32) for _ in srcs]
+        with A.Server(g, lanes=8) as srv:
+            ts = [srv.submit(A.Server.BFS, s_, o) for s_, o in zip(srcs, outs)]
+            for t in ts:
+                srv.wait(t)
+            st = srv.stats()
+        assert st["fallbacks"] >= 3
+        for s_, o in zip(srcs, outs):
+            assert np.array_equal(o, O.bfs_queue(csr, s_)), s_
+
+
 def test_served_values_follow_value_convention():
     h = A.rmat(12, 8, seed=4, weighted=True)
     off, ew = h.offsets().copy(), h.edge_words().copy()
@@ -205,7 +229,7 @@ def test_staggered_arrivals_from_threads(wgraph):
 def test_lanes_limited_by_memory_budget(wgraph):
     g, off, ew = wgraph
     V = len(off) - 1
-    budget = 2 * 2 * 4 * V  # two lanes per kind
+    budget = 2 * 3 * V  # two lanes per kind: V bytes per BFS lane, 2 V per SSSP lane
     srcs = [0, 5, 17, 99, 1000, 2047]
     outs = [np.empty(V, np.uint32) for _ in srcs]
     with A.Server(g, state_budget_bytes=budget) as srv:
