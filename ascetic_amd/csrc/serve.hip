@@ -120,6 +120,38 @@ __device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uin
     return atomicOr(a.Y + v, nw) == 0;
 }
 
+// msbfs_visit for a lane's kMsUnroll edges at once (edge j counts when bit j
+// of `valid` is set): the seen words of all of them are loaded, then their
+// seen atomics issued, then their next-frontier atomics, before any result is
+// used.  push[j]: v[j] enters the next frontier queue.
+constexpr int kMsUnroll = 4;
+
+template <class OffT, class EdgeT>
+__device__ __forceinline__ void msbfs_visit_batch(const BatchArgs<OffT, EdgeT>& a, const uint32_t (&v)[kMsUnroll],
+                                                  const unsigned long long (&bits)[kMsUnroll], uint32_t valid,
+                                                  uint32_t step, unsigned long long& act, bool (&push)[kMsUnroll]) {
+    unsigned long long nw[kMsUnroll], old[kMsUnroll];
+#pragma unroll
+    for (int j = 0; j < kMsUnroll; j++) nw[j] = ((valid >> j) & 1u) ? bits[j] & ~a.seen[v[j]] : 0ull;
+#pragma unroll
+    for (int j = 0; j < kMsUnroll; j++)
+        if (nw[j]) old[j] = atomicOr(a.seen + v[j], nw[j]);
+#pragma unroll
+    for (int j = 0; j < kMsUnroll; j++) {
+        push[j] = false;
+        if (!nw[j]) continue;
+        nw[j] &= ~old[j];
+        if (!nw[j]) continue;
+        act |= nw[j];
+        msbfs_set_levels(a, v[j], nw[j], step);
+    }
+#pragma unroll
+    for (int j = 0; j < kMsUnroll; j++)
+        if (nw[j]) old[j] = atomicOr(a.Y + v[j], nw[j]);
+#pragma unroll
+    for (int j = 0; j < kMsUnroll; j++) push[j] = nw[j] && old[j] == 0;
+}
+
 template <class OffT, class EdgeT>
 __global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a) {
     const uint32_t step = a.sp->step;
@@ -171,19 +203,25 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_local(BatchArgs<OffT, EdgeT> a
         s_bits[tid] = bits;
         __syncthreads();
         const uint32_t cnt = min(n - base, (uint32_t)kBlock);
-        for (uint32_t eb = w * kWave; eb < total; eb += kBlock) {
-            const uint32_t e = eb + lane;
-            bool push = false;
-            uint32_t v = 0;
-            if (e < total) {
+        for (uint32_t eb = w * kWave * kMsUnroll; eb < total; eb += kBlock * kMsUnroll) {
+            uint32_t v[kMsUnroll], valid = 0;
+            unsigned long long bb[kMsUnroll];
+#pragma unroll
+            for (int j = 0; j < kMsUnroll; j++) {
+                const uint32_t idx = eb + j * kWave + lane;
+                if (idx < total) valid |= 1u << j;
+                const uint32_t e = min(idx, total - 1);
                 uint32_t k = 0;
 #pragma unroll
                 for (uint32_t stp = kBlock / 2; stp > 0; stp >>= 1)
                     if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
-                v = edge_dst(a.edges[s_beg[k] + (OffT)(e - s_pref[k])]);
-                push = msbfs_visit(a, v, s_bits[k], step, act);
+                v[j] = edge_dst(a.edges[s_beg[k] + (OffT)(e - s_pref[k])]);
+                bb[j] = s_bits[k];
             }
-            st.push(push, v, a.q_out, &nxt->n_res);
+            bool push[kMsUnroll];
+            msbfs_visit_batch(a, v, bb, valid, step, act, push);
+#pragma unroll
+            for (int j = 0; j < kMsUnroll; j++) st.push(push[j], v[j], a.q_out, &nxt->n_res);
         }
         __syncthreads();
     }
@@ -229,20 +267,25 @@ __global__ __launch_bounds__(kBlock) void k_msbfs_heavy(BatchArgs<OffT, EdgeT> a
         }
         __syncthreads();
         const uint32_t tlen = (uint32_t)min((uint64_t)kTile, total - t0);
-        for (uint32_t r = w * kWave; r < tlen; r += kBlock) {
-            const uint32_t rr = r + lane;
-            bool push = false;
-            uint32_t v = 0;
-            if (rr < tlen) {
-                const uint64_t e = t0 + rr;
+        for (uint32_t r = w * kWave * kMsUnroll; r < tlen; r += kBlock * kMsUnroll) {
+            uint32_t v[kMsUnroll], valid = 0;
+            unsigned long long bb[kMsUnroll];
+#pragma unroll
+            for (int jj = 0; jj < kMsUnroll; jj++) {
+                const uint32_t rr = r + jj * kWave + lane;
+                if (rr < tlen) valid |= 1u << jj;
+                const uint64_t e = t0 + min(rr, tlen - 1);
                 uint32_t k = 0;
 #pragma unroll
                 for (uint32_t j = 1; j < (uint32_t)kWin; j++)
                     if (j < nwin && s_hb[j] <= e) k = j;
-                v = edge_dst(a.edges[s_hoff[k] + (OffT)(e - s_hb[k])]);
-                push = msbfs_visit(a, v, s_hbits[k], step, act);
+                v[jj] = edge_dst(a.edges[s_hoff[k] + (OffT)(e - s_hb[k])]);
+                bb[jj] = s_hbits[k];
             }
-            st.push(push, v, a.q_out, &nxt->n_res);
+            bool push[kMsUnroll];
+            msbfs_visit_batch(a, v, bb, valid, step, act, push);
+#pragma unroll
+            for (int jj = 0; jj < kMsUnroll; jj++) st.push(push[jj], v[jj], a.q_out, &nxt->n_res);
         }
         __syncthreads();
     }
