@@ -453,8 +453,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one MI355X only: this build has no multi-GPU path "
-              "(see DESIGN.md for how the traversal would shard); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on one MI355X only: the library has no multi-GPU path (a 1-D vertex partition "
+              "with an all-to-all of relaxations per step is the intended sharding); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     import ascetic_amd as A
