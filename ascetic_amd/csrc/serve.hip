@@ -107,23 +107,12 @@ __device__ __forceinline__ void msbfs_set_levels(const BatchArgs<OffT, EdgeT>& a
     if (over) atomicOr(a.ovf, over);
 }
 
-template <class OffT, class EdgeT>
-__device__ __forceinline__ bool msbfs_visit(const BatchArgs<OffT, EdgeT>& a, uint32_t v, unsigned long long bits,
-                                            uint32_t step, unsigned long long& act) {
-    unsigned long long nw = bits & ~a.seen[v];
-    if (!nw) return false;
-    const unsigned long long old = atomicOr(a.seen + v, nw);
-    nw &= ~old;
-    if (!nw) return false;
-    act |= nw;
-    msbfs_set_levels(a, v, nw, step);
-    return atomicOr(a.Y + v, nw) == 0;
-}
-
-// msbfs_visit for a lane's kMsUnroll edges at once (edge j counts when bit j
-// of `valid` is set): the seen words of all of them are loaded, then their
-// seen atomics issued, then their next-frontier atomics, before any result is
-// used.  push[j]: v[j] enters the next frontier queue.
+// Top-down visit of kMsUnroll edges per lane (edge j counts when bit j of
+// `valid` is set): for each destination v, the lanes of `bits` not yet in
+// seen[v] claim v (one atomic OR arbitrates each lane), record its level and
+// enter its next-frontier word.  The seen words of all edges are loaded, then
+// their seen atomics issued, then their next-frontier atomics, before any
+// result is used.  push[j]: v[j] enters the next frontier queue.
 constexpr int kMsUnroll = 4;
 
 template <class OffT, class EdgeT>
