@@ -19,46 +19,26 @@ __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
 // Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
 // and routing of an improved vertex, in two steps so that a wave can issue
 // the probes of several edges before any of their atomics: relax_probe loads
-// the destination's word (BFS: visited bits; SSSP/CC: value), relax_commit
-// decides from it and performs the atomics.
+// the destination's word (BFS: visited bits; SSSP/CC: value),
+// relax_commit_batch decides from it and performs the atomics.
 template <int ALGO, class OffT, class EdgeT>
 __device__ __forceinline__ uint32_t relax_probe(const TravArgs<OffT, EdgeT>& a, uint32_t v) {
     return ALGO == ALG_BFS ? a.visited[v >> 5] : a.val[v];
-}
-
-template <int ALGO, class OffT, class EdgeT>
-__device__ __forceinline__ int relax_commit(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level,
-                                            uint32_t probe) {
-    if (ALGO == ALG_BFS) {
-        const uint32_t bit = 1u << (v & 31);
-        if (probe & bit) return TO_NONE;
-        if (atomicOr(a.visited + (v >> 5), bit) & bit) return TO_NONE;
-        a.val[v] = cand;
-        return v < a.vs ? TO_RES : TO_PEND;
-    } else {
-        if (cand >= probe) return TO_NONE;
-        const uint32_t old = atomicMin(a.val + v, cand);
-        if (old <= cand) return TO_NONE;
-        if (ALGO == ALG_SSSP && cand >= a.theta)
-            return atomicMax(a.fstamp + v, a.fepoch) < a.fepoch ? TO_FAR : TO_NONE;
-        if (v < a.vs) return atomicMax(a.stamp + v, level + 1) < level + 1 ? TO_RES : TO_NONE;
-        return atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
-    }
-}
-
-template <int ALGO, class OffT, class EdgeT>
-__device__ __forceinline__ int relax(const TravArgs<OffT, EdgeT>& a, uint32_t v, uint32_t cand, uint32_t level) {
-    return relax_commit<ALGO>(a, v, cand, level, relax_probe<ALGO>(a, v));
 }
 
 // Edges each lane takes per pass of an edge loop: the loads of kUnroll edges,
 // then their probes, are in flight together.
 constexpr int kUnroll = 4;
 
-// relax_commit for a lane's kUnroll edges at once (edge j counts when bit j
-// of `valid` is set): the value atomics of all of them are issued before any
-// result is used, then the stamp atomics of those that improved, so each
-// lane waits for two atomic round trips per pass instead of two per edge.
+// The commit of a lane's kUnroll edges (edge j counts when bit j of `valid`
+// is set).  BFS: a vertex whose visited bit is clear is claimed by an atomic
+// OR of that bit and gets its level; SSSP/CC: a candidate below the probed
+// value is offered by atomicMin, and a vertex it lowers is routed once per
+// step (a far candidate to the far pile, a static vertex to the resident
+// frontier, an on-demand one to the pile) by an atomicMax on the matching
+// stamp.  The value atomics of all edges are issued before any result is
+// used, then the stamp atomics of those that improved, so each lane waits for
+// two atomic round trips per pass instead of two per edge.
 template <int ALGO, class OffT, class EdgeT>
 __device__ __forceinline__ void relax_commit_batch(const TravArgs<OffT, EdgeT>& a, const uint32_t (&v)[kUnroll],
                                                    const uint32_t (&cand)[kUnroll], const uint32_t (&probe)[kUnroll],
