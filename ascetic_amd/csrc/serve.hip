@@ -436,9 +436,9 @@ __device__ __forceinline__ uint32_t lanes_changed(unsigned long long x, unsigned
 // beat the value read, one compare-and-swap per 64-bit word of four lanes.
 // Returns true when v must be queued.
 template <class OffT>
-__device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint32_t u, uint32_t v, uint32_t wgt,
-                                           unsigned long long bits, unsigned long long& act) {
-    const uint4* du4 = (const uint4*)(a.dist + (size_t)u * a.stride);
+__device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, const uint16_t* du_row, uint32_t v,
+                                           uint32_t wgt, unsigned long long bits, unsigned long long& act) {
+    const uint4* du4 = (const uint4*)du_row;
     const uint4* dv4 = (const uint4*)(a.dist + (size_t)v * a.stride);
     const uint32_t w2 = (wgt >= kD16Inf ? kD16Inf : wgt) * 0x10001u;   // the weight in both halves
     unsigned long long maybe = 0, over = 0;
@@ -480,7 +480,7 @@ __device__ __forceinline__ bool mssp_relax(const BatchArgs<OffT, WEdge>& a, uint
     // phase 2, per 64-bit word of four lanes: one compare-and-swap lowers
     // every lane of the word whose candidate beats it; the compare-and-swaps
     // of kRW words are in flight together, a lost race is retried alone
-    const unsigned long long* du64 = (const unsigned long long*)(a.dist + (size_t)u * a.stride);
+    const unsigned long long* du64 = (const unsigned long long*)du_row;
     unsigned long long* row = (unsigned long long*)(a.dist + (size_t)v * a.stride);
     unsigned long long imp = 0;
     for (uint32_t wb = 0; wb < kMaxLanes / 4; wb += kRW) {
@@ -540,10 +540,10 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a)
     const uint32_t step = a.sp->step;
     __shared__ uint32_t s_pref[kBlock];
     __shared__ OffT s_beg[kBlock];
-    __shared__ uint32_t s_u[kBlock];
     __shared__ unsigned long long s_bits[kBlock];
     __shared__ uint32_t s_tmp[kWavesPerBlock];
     __shared__ uint32_t s_stage[kWavesPerBlock][kStageCap];
+    __shared__ uint4 s_rows[kBlock][kMaxLanes / 8];   // the tile's source rows (read by every edge of a vertex)
     Slot* cur = a.slots + (step & 3);
     Slot* nxt = a.slots + ((step + 1) & 3);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -554,6 +554,7 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a)
     }
     const uint32_t n = cur->n_res;
     const uint32_t tid = threadIdx.x, w = wave_id();
+    const uint32_t row_u4 = a.stride / 8;
     WaveStage<kStageCap> st{s_stage[w], 0};
     unsigned long long act = 0;
     for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
@@ -575,13 +576,18 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a)
                 a.heavy_base[idx] = p & kHeavyMask;
             } else {
                 deg = (uint32_t)d;
+                if (deg) {
+                    // u's row as of the tile's start: a value lowered later in
+                    // the step queues u again, so an older (larger) one is safe
+                    const uint4* src = (const uint4*)(a.dist + (size_t)u * a.stride);
+                    for (uint32_t g = 0; g < row_u4; g++) s_rows[tid][g] = src[g];
+                }
             }
         }
         uint32_t total;
         const uint32_t pre = block_scan_excl(deg, s_tmp, total);
         s_pref[tid] = pre;
         s_beg[tid] = beg;
-        s_u[tid] = u;
         s_bits[tid] = bits;
         __syncthreads();
         const uint32_t cnt = min(n - base, (uint32_t)kBlock);
@@ -596,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_mssp_local(BatchArgs<OffT, WEdge> a)
                     if (k + stp < cnt && s_pref[k + stp] <= e) k += stp;
                 const WEdge ed = a.edges[s_beg[k] + (OffT)(e - s_pref[k])];
                 v = ed.dst;
-                push = mssp_relax(a, s_u[k], v, ed.w, s_bits[k], act);
+                push = mssp_relax(a, (const uint16_t*)s_rows[k], v, ed.w, s_bits[k], act);
             }
             st.push(push, v, a.q_out, &nxt->n_res);
         }
@@ -659,7 +665,7 @@ __global__ __launch_bounds__(kBlock) void k_mssp_heavy(BatchArgs<OffT, WEdge> a)
                     if (j < nwin && s_hb[j] <= e) k = j;
                 const WEdge ed = a.edges[s_hoff[k] + (OffT)(e - s_hb[k])];
                 v = ed.dst;
-                push = mssp_relax(a, s_hu[k], v, ed.w, s_hbits[k], act);
+                push = mssp_relax(a, a.dist + (size_t)s_hu[k] * a.stride, v, ed.w, s_hbits[k], act);
             }
             st.push(push, v, a.q_out, &nxt->n_res);
         }
