@@ -1,0 +1,75 @@
+"""Hand-derived known answers (tests/golden/known_answers.json): the oracle on
+the CPU, and the library on the GPU through the C ABI."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import ascetic_amd as A
+from ascetic_amd import _abi
+from oracle import oracle as O
+from tests import graphs as G
+
+with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "known_answers.json")) as f:
+    CASES = json.load(f)["cases"]
+
+
+def build(case):
+    e = case["edges"]
+    weighted = len(e[0]) == 3
+    off, dst, w = G.csr_from_edges(case["V"], [(x[0], x[1]) for x in e], [x[2] for x in e] if weighted else None,
+                                   symmetric=case["symmetric"])
+    return off, dst, w
+
+
+def oracle_answer(case):
+    off, dst, w = build(case)
+    V = case["V"]
+    if case["algo"] == "pagerank":
+        ioff, isrc = G.transpose(off, dst, V)
+        r, _ = O.pr_norm(O.CSR(ioff, isrc), np.diff(off).astype(np.uint32), case["damping"], case["iterations"])
+        return r
+    csr = O.CSR(off, dst, w)
+    if case["algo"] == "bfs":
+        out = O.bfs_queue(csr, case["source"])
+    elif case["algo"] == "sssp":
+        out = O.dijkstra(csr, case["source"])
+    else:
+        out = O.cc_minreach(csr)
+    if case.get("convention") == "zero":
+        out = np.where(out == V + 1, 0xFFFFFFFF, out - 1).astype(np.uint32)
+    return out
+
+
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_oracle_matches_known_answer(case):
+    got = oracle_answer(case)
+    if case["algo"] == "pagerank":
+        assert np.allclose(got, case["expected"], rtol=0, atol=1e-12)
+    else:
+        assert got.tolist() == case["expected"]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])
+def test_library_matches_known_answer(case):
+    off, dst, w = build(case)
+    V = case["V"]
+    conv = _abi.VALUES_ZERO_BASED if case.get("convention") == "zero" else _abi.VALUES_REFERENCE
+    with A.Context(value_convention=conv) as ctx:
+        if case["algo"] == "pagerank":
+            ioff, isrc = G.transpose(off, dst, V)
+            with ctx.from_arrays(ioff, isrc, out_degree=np.diff(off).astype(np.uint32)) as g:
+                r, _, _ = g.pagerank(iterations=case["iterations"], damping=case["damping"])
+            assert float(np.abs(r.astype(np.float64) - np.array(case["expected"])).sum()) <= 1e-5
+            return
+        edges = dst if w is None else G.wedges(dst, w)
+        with ctx.from_arrays(off, edges) as g:
+            if case["algo"] == "bfs":
+                out, _ = g.bfs(case["source"])
+            elif case["algo"] == "sssp":
+                out, _ = g.sssp(case["source"])
+            else:
+                out, _ = g.cc()
+        assert out.tolist() == case["expected"]
