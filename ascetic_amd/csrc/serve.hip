@@ -738,11 +738,15 @@ __global__ void k_lane_fill(uint32_t* p, uint64_t n, uint32_t v) {
         p[i] = v;
 }
 
-// Result column of one lane, unreached as V + 1.
+// Result column of one lane, unreached as V + 1.  The lanes relax without the
+// reference's cap: there a value starts at V + 1 and only a smaller candidate
+// lowers it (GraphMeta.cuh:440-446, gpu_kernels.cuh:499), so a vertex whose
+// distance + 1 reaches V + 1 keeps V + 1.  With non-negative weights that
+// schedule ends at min(exact value, V + 1), which is applied here.
 __global__ void k_extract_lane(const uint16_t* dist, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
     for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t d = dist[v * stride + lane];
-        out[v] = d == kD16Inf ? (uint32_t)(V + 1) : d;
+        out[v] = d == kD16Inf || (uint64_t)d > V ? (uint32_t)(V + 1) : d;
     }
 }
 

@@ -82,3 +82,37 @@ def test_library_matches_known_answer(case, on_demand):
             else:
                 out, _ = g.cc()
         assert out.tolist() == case["expected"]
+
+
+SERVED = [c for c in CASES if c["algo"] in ("bfs", "sssp")]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("step_graphs", [True, False], ids=["graphs", "eager"])
+@pytest.mark.parametrize("case", SERVED, ids=[c["name"] for c in SERVED])
+def test_served_matches_known_answer(case, step_graphs):
+    """The serving engine answers each case exactly, alone and mixed with
+    queries from every vertex (several copies of the case's query among them),
+    including the cases whose values reach the V+1 cap."""
+    off, dst, w = build(case)
+    V = case["V"]
+    conv = _abi.VALUES_ZERO_BASED if case.get("convention") == "zero" else _abi.VALUES_REFERENCE
+    kind = A.Server.BFS if case["algo"] == "bfs" else A.Server.SSSP
+    print("served case", case["name"], "step_graphs", step_graphs, flush=True)
+    with A.Context(value_convention=conv) as ctx:
+        edges = dst if w is None else G.wedges(dst, w)
+        with ctx.from_arrays(off, edges) as g:
+            with A.Server(g, lanes=16, step_graphs=step_graphs) as srv:
+                alone = np.empty(V, np.uint32)
+                srv.wait(srv.submit(kind, case["source"], alone))
+                assert alone.tolist() == case["expected"]
+                mix = [case["source"]] * 3 + list(range(V))
+                outs = [np.empty(V, np.uint32) for _ in mix]
+                tickets = [srv.submit(kind, s, o) for s, o in zip(mix, outs)]
+                for t in tickets:
+                    srv.wait(t)
+            for s, o in zip(mix, outs):
+                ref, _ = (g.bfs(s) if kind == A.Server.BFS else g.sssp(s))
+                assert np.array_equal(o, ref), (case["name"], s)
+            for o in outs[:3]:
+                assert o.tolist() == case["expected"]
