@@ -11,9 +11,13 @@ namespace asc {
 
 enum { TO_NONE = 0, TO_RES = 1, TO_PEND = 2, TO_FAR = 3 };
 
+// SSSP: value + weight, saturating at 0xFFFFFFFF (one clamped add), so a
+// weight near 2^32 cannot wrap to a small candidate; every saturated candidate
+// is above V + 1 and lowers nothing, which leaves min(exact, V + 1) as the
+// final value.  BFS passes level + 2 as sv, CC the label.
 template <int ALGO>
 __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
-    return ALGO == ALG_SSSP ? sv + w : sv;  // BFS passes level + 2 as sv
+    return ALGO == ALG_SSSP ? __builtin_elementwise_add_sat(sv, w) : sv;
 }
 
 // Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)

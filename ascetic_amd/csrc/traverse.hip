@@ -609,7 +609,7 @@ static uint32_t sssp_delta(asc_graph* g) {
     // re-expansions wide bands cause); fully resident ones narrow bands (fewer
     // re-expansions; a step costs only a launch and a sync)
     const double f = g->static_vertices == g->h.V ? 0.25 : 6.0;
-    return std::max<uint32_t>(1, (uint32_t)(f * g->mean_weight));
+    return (uint32_t)std::min(std::max(1.0, f * g->mean_weight), 2147483648.0);
 }
 
 // Copies the adjacency of list entries overlapping [lo, hi) of the
