@@ -188,29 +188,21 @@ int asc_copy_probe(asc_ctx* ctx, uint64_t bytes, double* ms) {
     if (!ctx || bytes < 16) return ASC_ERR_INVALID;
     ASC_HIP(hipSetDevice(ctx->device));
     const uint64_t n = bytes / 16;
-    void *src = nullptr, *dst = nullptr;
-    if (hipMalloc(&src, 16 * n) != hipSuccess) return ASC_ERR_NOMEM;
-    if (hipMalloc(&dst, 16 * n) != hipSuccess) {
-        (void)hipFree(src);
-        return ASC_ERR_NOMEM;
-    }
-    hipEvent_t a, b;
-    ASC_HIP(hipEventCreate(&a));
-    ASC_HIP(hipEventCreate(&b));
-    ASC_HIP(hipMemsetAsync(src, 1, 16 * n, ctx->s_main));
-    ASC_HIP(hipEventRecord(a, ctx->s_main));
-    hipLaunchKernelGGL(k_copy_probe, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->s_main, (const uint4*)src, (uint4*)dst,
+    DevTmp<uint4> src, dst;
+    if (hipMalloc(&src.p, 16 * n) != hipSuccess || hipMalloc(&dst.p, 16 * n) != hipSuccess) return ASC_ERR_NOMEM;
+    ScopedEvent a, b;
+    ASC_HIP(hipEventCreate(&a.e));
+    ASC_HIP(hipEventCreate(&b.e));
+    ASC_HIP(hipMemsetAsync(src.p, 1, 16 * n, ctx->s_main));
+    ASC_HIP(hipEventRecord(a.e, ctx->s_main));
+    hipLaunchKernelGGL(k_copy_probe, dim3(ctx->num_cus * 8), dim3(256), 0, ctx->s_main, (const uint4*)src.p, dst.p,
                        n);
     ASC_HIP(hipGetLastError());
-    ASC_HIP(hipEventRecord(b, ctx->s_main));
-    ASC_HIP(hipEventSynchronize(b));
+    ASC_HIP(hipEventRecord(b.e, ctx->s_main));
+    ASC_HIP(hipEventSynchronize(b.e));
     float t = 0;
-    (void)hipEventElapsedTime(&t, a, b);
+    (void)hipEventElapsedTime(&t, a.e, b.e);
     if (ms) *ms = t;
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipFree(src);
-    (void)hipFree(dst);
     return ASC_OK;
 }
 
@@ -274,31 +266,28 @@ int asc_measure_link(asc_ctx* ctx, uint64_t bytes, int reps, double* h2d_gbps, d
     HostBuf<char> host;
     ASC_TRY(host.alloc(bytes, true));
     std::memset(host.p, 1, bytes);
-    void* dev = nullptr;
-    if (hipMalloc(&dev, bytes) != hipSuccess) return ASC_ERR_NOMEM;
-    hipEvent_t a, b;
+    DevTmp<char> dev;
+    if (hipMalloc(&dev.p, bytes) != hipSuccess) return ASC_ERR_NOMEM;
+    ScopedEvent a, b;
     int rc = ASC_OK;
-    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = ASC_ERR_DEVICE;
+    if (hipEventCreate(&a.e) != hipSuccess || hipEventCreate(&b.e) != hipSuccess) rc = ASC_ERR_DEVICE;
     for (int dir = 0; dir < 2 && rc == ASC_OK; dir++) {
         auto copy = [&] {
-            return dir == 0 ? hipMemcpyAsync(dev, host.p, bytes, hipMemcpyHostToDevice, ctx->s_copy)
-                            : hipMemcpyAsync(host.p, dev, bytes, hipMemcpyDeviceToHost, ctx->s_copy);
+            return dir == 0 ? hipMemcpyAsync(dev.p, host.p, bytes, hipMemcpyHostToDevice, ctx->s_copy)
+                            : hipMemcpyAsync(host.p, dev.p, bytes, hipMemcpyDeviceToHost, ctx->s_copy);
         };
         if (copy() != hipSuccess) rc = ASC_ERR_DEVICE;  // warm-up
-        (void)hipEventRecord(a, ctx->s_copy);
+        (void)hipEventRecord(a.e, ctx->s_copy);
         for (int i = 0; i < reps && rc == ASC_OK; i++)
             if (copy() != hipSuccess) rc = ASC_ERR_DEVICE;
-        (void)hipEventRecord(b, ctx->s_copy);
-        (void)hipEventSynchronize(b);
+        (void)hipEventRecord(b.e, ctx->s_copy);
+        (void)hipEventSynchronize(b.e);
         float ms = 0;
-        (void)hipEventElapsedTime(&ms, a, b);
+        (void)hipEventElapsedTime(&ms, a.e, b.e);
         const double gbps = ms > 0 ? (double)bytes * reps / (ms * 1e-3) / 1e9 : 0;
         if (dir == 0 && h2d_gbps) *h2d_gbps = gbps;
         if (dir == 1 && d2h_gbps) *d2h_gbps = gbps;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    (void)hipFree(dev);
     return rc;
 }
 

@@ -98,6 +98,30 @@ struct HostBuf {
     const T& operator[](size_t i) const { return p[i]; }
 };
 
+// A temporary device allocation of set-up code, freed when it leaves scope
+// (early error returns included).
+template <class T>
+struct DevTmp {
+    T* p = nullptr;
+    DevTmp() = default;
+    DevTmp(const DevTmp&) = delete;
+    DevTmp& operator=(const DevTmp&) = delete;
+    ~DevTmp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+// A timing event destroyed when it leaves scope.
+struct ScopedEvent {
+    hipEvent_t e = nullptr;
+    ScopedEvent() = default;
+    ScopedEvent(const ScopedEvent&) = delete;
+    ScopedEvent& operator=(const ScopedEvent&) = delete;
+    ~ScopedEvent() {
+        if (e) (void)hipEventDestroy(e);
+    }
+};
+
 // Host CSR: offsets with sentinel (V+1), edge words (E, or 2E for {dst,w}).
 struct HostCSR {
     uint64_t V = 0, E = 0;

@@ -556,35 +556,35 @@ static int pr_blocked_build(asc_graph* g, PrState* p) {
         set_last_error("out-degrees sum to %llu, not E = %llu", (unsigned long long)h_out[V], (unsigned long long)E);
         return ASC_ERR_FORMAT;
     }
-    unsigned long long* d_out_off = nullptr;
-    uint32_t* out_dst = sym ? (uint32_t*)g->d_edges : nullptr;
-    ASC_HIP(hipMalloc(&d_out_off, 8 * (V + 1)));
-    ASC_HIP(hipMemcpy(d_out_off, h_out.p, 8 * (V + 1), hipMemcpyHostToDevice));
+    // set-up temporaries (freed on every return): out-offsets, the transpose
+    // of a directed graph and its fill cursors, the slot counts
+    DevTmp<unsigned long long> d_out_off, cursor;
+    DevTmp<uint32_t> transposed, d_cnt;
+    ASC_HIP(hipMalloc(&d_out_off.p, 8 * (V + 1)));
+    ASC_HIP(hipMemcpy(d_out_off.p, h_out.p, 8 * (V + 1), hipMemcpyHostToDevice));
+    const uint32_t* out_dst = (const uint32_t*)g->d_edges;
     if (!sym) {
-        unsigned long long* cursor = nullptr;
-        ASC_HIP(hipMalloc(&out_dst, 4 * E));
-        ASC_HIP(hipMalloc(&cursor, 8 * (V + 1)));
-        ASC_HIP(hipMemcpyAsync(cursor, d_out_off, 8 * (V + 1), hipMemcpyDeviceToDevice, s));
+        ASC_HIP(hipMalloc(&transposed.p, 4 * E));
+        ASC_HIP(hipMalloc(&cursor.p, 8 * (V + 1)));
+        ASC_HIP(hipMemcpyAsync(cursor.p, d_out_off.p, 8 * (V + 1), hipMemcpyDeviceToDevice, s));
         hipLaunchKernelGGL((k_pb_transpose<OffT>), dim3(grid), dim3(kBlock), 0, s, in_off,
-                           (const uint32_t*)g->d_edges, V, cursor, out_dst);
+                           (const uint32_t*)g->d_edges, V, cursor.p, transposed.p);
         ASC_HIP(hipGetLastError());
         ASC_HIP(hipStreamSynchronize(s));
-        (void)hipFree(cursor);
+        out_dst = transposed.p;
     }
     ASC_HIP(hipMalloc(&p->pb_src, 4 * E));
-    hipLaunchKernelGGL((k_pb_src<unsigned long long>), dim3(grid), dim3(kBlock), 0, s, d_out_off, V, p->pb_src);
+    hipLaunchKernelGGL((k_pb_src<unsigned long long>), dim3(grid), dim3(kBlock), 0, s, d_out_off.p, V, p->pb_src);
     ASC_HIP(hipGetLastError());
     // slot layout: counts per (bin, range), exclusive scan in bin-major order
     const uint32_t thr = (uint32_t)p->heavy_thr;
-    uint32_t* d_cnt = nullptr;
-    ASC_HIP(hipMalloc(&d_cnt, 4 * (size_t)kPbRanges * nb));
+    ASC_HIP(hipMalloc(&d_cnt.p, 4 * (size_t)kPbRanges * nb));
     hipLaunchKernelGGL((k_pb_count<OffT>), dim3(std::min<uint32_t>(kPbRanges, grid)), dim3(kBlock), 0, s, out_dst, E,
-                       in_off, thr, (uint32_t)nb, d_cnt);
+                       in_off, thr, (uint32_t)nb, d_cnt.p);
     ASC_HIP(hipGetLastError());
     std::vector<uint32_t> cnt((size_t)kPbRanges * nb);
-    ASC_HIP(hipMemcpyAsync(cnt.data(), d_cnt, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
     ASC_HIP(hipStreamSynchronize(s));
-    (void)hipFree(d_cnt);
     // range-major slots: the regions of one source range are adjacent, so
     // the scatter of a range writes one contiguous stretch
     std::vector<unsigned long long> base(cnt.size());
@@ -605,8 +605,6 @@ static int pr_blocked_build(asc_graph* g, PrState* p) {
                        E, in_off, thr, (uint32_t)nb, p->pb_bin_start, p->pb_pos, p->pb_dl);
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipStreamSynchronize(s));
-    (void)hipFree(d_out_off);
-    if (!sym) (void)hipFree(out_dst);
     p->pb_nbins = (uint32_t)nb;
     p->pb_edges = E;
     p->pb_state = 1;
@@ -635,11 +633,11 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         set_last_error("propagation-blocked PageRank needs every edge resident, E < 2^32 and room for its slots");
         return ASC_ERR_UNSUPPORTED;
     }
-    hipEvent_t e0, e1;
-    ASC_HIP(hipEventCreate(&e0));
-    ASC_HIP(hipEventCreate(&e1));
+    ScopedEvent e0, e1;
+    ASC_HIP(hipEventCreate(&e0.e));
+    ASC_HIP(hipEventCreate(&e1.e));
     const double t0 = now_ms();
-    ASC_HIP(hipEventRecord(e0, s));
+    ASC_HIP(hipEventRecord(e0.e, s));
     for (int it = 0; it < iters; it++) {
         PrArgs a{};
         a.off = g->d_off;
@@ -687,13 +685,11 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
             return launch_pull<OffT>(G, cgrid, s, c);
         }));
     }
-    ASC_HIP(hipEventRecord(e1, s));
+    ASC_HIP(hipEventRecord(e1.e, s));
     ASC_HIP(hipStreamSynchronize(s));
     const double t1 = now_ms();
     float kms = 0;
-    (void)hipEventElapsedTime(&kms, e0, e1);
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    (void)hipEventElapsedTime(&kms, e0.e, e1.e);
     std::vector<double> hres(iters);
     ASC_HIP(hipMemcpy(hres.data(), resid, sizeof(double) * iters, hipMemcpyDeviceToHost));
     if (residual_out) std::memcpy(residual_out, hres.data(), sizeof(double) * iters);

@@ -1258,9 +1258,9 @@ int asc_server_destroy(asc_server* s) {
     if (s->th.joinable()) s->th.join();
     engine_free(s->eng[0]);
     engine_free(s->eng[1]);
+    // the ticket map owns every query not yet waited for (pending ones too;
+    // the loop drains the queues before it ends)
     for (auto& kv : s->tickets) delete kv.second;
-    for (int k = 0; k < 2; k++)
-        for (SQuery* q : s->pending[k]) delete q;
     delete s;
     return ASC_OK;
 }
