@@ -267,7 +267,7 @@ def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads
 def stream_section(A, _abi, args):
     """The main line: SSSP on the budget-capped R-MAT graph, streamed."""
     t = time.perf_counter()
-    h = A.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=True, wmin=1, wmax=64)
+    h = A.rmat(args.scale, args.edge_factor, seed=args.seed, weighted=True, wmin=1, wmax=args.wmax)
     off = h.offsets()
     ew = h.edge_words()
     V, E = h.num_vertices, h.num_edges
@@ -311,7 +311,7 @@ def stream_section(A, _abi, args):
     link_h2d, link_d2h = ctx.measure_link(256 << 20, 8)
     if args.copy_probe_mb > 0:
         ctx.copy_probe(args.copy_probe_mb << 20)
-    workload = f"sssp_rmat_s{args.scale}_ef{args.edge_factor}_w1-64_budget{int(args.budget_frac * 100)}"
+    workload = f"sssp_rmat_s{args.scale}_ef{args.edge_factor}_w1-{args.wmax}_budget{int(args.budget_frac * 100)}"
     # HBM bytes per launch of the dominant kernel, from the committed rocprofv3
     # FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated on a copy)
     traffic = None
@@ -338,7 +338,7 @@ def stream_section(A, _abi, args):
         "config": {
             "workload": workload,
             "graph": f"R-MAT scale {args.scale}, edge factor {args.edge_factor}, (0.57,0.19,0.19,0.05), "
-                     f"symmetrized, seed {args.seed}",
+                     f"symmetrized, weights uniform in [1, {args.wmax}], seed {args.seed}",
             "V": V, "E_directed": E, "source": src, "offsets": "u64", "edge_record": "{u32 dst, u32 w}",
             "edge_budget_bytes": budget, "static_edges": info["static_edges"],
             "ondemand_edges": info["ondemand_edges"], "part_bytes": args.part_mb << 20,
@@ -431,6 +431,7 @@ def main():
     ap.add_argument("--scale", type=int, default=24, help="R-MAT scale of the main streamed line (0 = skip)")
     ap.add_argument("--edge-factor", type=int, default=16)
     ap.add_argument("--seed", type=int, default=24)
+    ap.add_argument("--wmax", type=int, default=64, help="edge weights uniform in [1, wmax]")
     ap.add_argument("--budget-frac", type=float, default=0.4)
     ap.add_argument("--part-mb", type=int, default=64)
     ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
