@@ -388,13 +388,17 @@ class Server:
         check(lib.asc_server_create(graph._h, C.byref(o), C.byref(h)), "asc_server_create")
         self._h = h
         self.graph = graph
+        self._V = graph.num_vertices
         self._outs = {}
 
     def submit(self, kind, source, out=None):
         t = C.c_uint64()
         ptr = None
         if out is not None:
-            assert out.dtype == np.uint32 and out.flags["C_CONTIGUOUS"]
+            # the library writes V values into the buffer from its own thread
+            if not (isinstance(out, np.ndarray) and out.dtype == np.uint32 and out.flags["C_CONTIGUOUS"]
+                    and out.flags["WRITEABLE"] and out.size >= self._V):
+                raise ValueError(f"out must be a writeable C-contiguous uint32 array of at least {self._V} entries")
             ptr = _u32p(out)
         check(_abi.load().asc_submit(self._h, kind, source, ptr, C.byref(t)), "asc_submit")
         if out is not None:
