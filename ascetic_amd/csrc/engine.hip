@@ -29,7 +29,9 @@ void graph_free_device(asc_graph* g) {
     g->pr = nullptr;
     g->zc_tail.release();
     g->ptail.release();
+    g->ptail_w.release();
     g->packed = false;
+    g->pk_split = false;
 }
 
 static int param_size_for(int kind) {
@@ -86,8 +88,11 @@ int graph_upload(asc_graph* g) {
         ASC_HIP(hipMemcpy(g->d_outdeg, g->h.outdeg.p, 4 * V, hipMemcpyHostToDevice));
         g->device_bytes += 4 * V;
     }
-    // packed on-demand stream: one u32 per edge when the id and weight fit
+    // packed on-demand stream: one u32 per edge when the id and weight fit,
+    // else a u32 id and a weight byte when the weights span at most 256 values
+    // (stream_pack 2 forces the split form)
     g->packed = false;
+    g->pk_split = false;
     if (ctx->cfg.stream_pack && g->h.kind == ASC_KIND_WCSR && g->static_edges < E) {
         const uint64_t S = g->static_edges, n = E - S;
         const uint32_t* ew = g->h.edges.p;
@@ -107,7 +112,8 @@ int graph_upload(asc_graph* g) {
         const uint32_t dbits = V > 1 ? 64 - (uint32_t)__builtin_clzll(V - 1) : 1;
         const uint32_t span = whi - wlo;
         const uint32_t wbits = span ? 32 - (uint32_t)__builtin_clz(span) : 0;
-        if (dbits + wbits <= 32 && dbits < 32) {
+        const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
+        if (dbits + wbits <= 32 && dbits < 32 && ctx->cfg.stream_pack != 2) {
             ASC_TRY(g->ptail.alloc(n, true));
             host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
                 for (uint64_t i = b; i < en; i++)
@@ -116,8 +122,21 @@ int graph_upload(asc_graph* g) {
             g->packed = true;
             g->pk_shift = dbits;
             g->pk_wmin = wlo;
-            const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
             g->part_edges = std::max<uint64_t>(pb / 4, 1024);
+        } else if (wbits <= 8) {
+            ASC_TRY(g->ptail.alloc(n, true));
+            ASC_TRY(g->ptail_w.alloc(n, true));
+            host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
+                for (uint64_t i = b; i < en; i++) {
+                    g->ptail[i] = ew[2 * (S + i)];
+                    g->ptail_w[i] = (uint8_t)(ew[2 * (S + i) + 1] - wlo);
+                }
+            });
+            g->packed = true;
+            g->pk_split = true;
+            g->pk_shift = 0;
+            g->pk_wmin = wlo;
+            g->part_edges = std::max<uint64_t>(pb / 5, 1024);
         }
     }
     // zero-copy streaming reads the on-demand edges from mapped pinned memory;

@@ -211,8 +211,16 @@ struct asc_graph {
     // when the id and weight bits fit (weighted graphs only); pinned.  The
     // streamed parts then carry 4 instead of 8 bytes per edge.
     asc::HostBuf<uint32_t> ptail;
+    // Split form, when the id and weight bits do not fit one word but the
+    // weights span at most 256 values: ptail holds the ids and ptail_w the
+    // bytes w - wmin (5 bytes per streamed edge).  A device part buffer or
+    // staging buffer of PE edges then holds PE ids followed by PE weight bytes.
+    asc::HostBuf<uint8_t> ptail_w;
     bool packed = false;
+    bool pk_split = false;
     uint32_t pk_shift = 32, pk_wmin = 0;
+    // bytes per streamed on-demand edge
+    size_t stream_record_bytes() const { return packed ? (pk_split ? 5 : 4) : h.edge_bytes(); }
     // tunables
     int heavy_threshold = 0;
     int pr_group = 0;

@@ -133,9 +133,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
                 for (uint32_t step = kTile / 2; step > 0; step >>= 1)
                     if (k + step < m && s_p[k + step] <= e) k += step;
                 if (PK) {
-                    const uint32_t x = ZC ? p.zc32[s_src[k] + (e - s_p[k])] : p.part32[e - p.lo];
-                    vv[j] = x & p.pk_mask;
-                    ww[j] = (x >> p.pk_shift) + p.pk_wmin;
+                    const uint64_t ix = ZC ? s_src[k] + (e - s_p[k]) : e - p.lo;
+                    const uint32_t x = ZC ? p.zc32[ix] : p.part32[ix];
+                    if (p.pk_split) {
+                        vv[j] = x;
+                        ww[j] = (uint32_t)(ZC ? p.zc_w8[ix] : p.part_w8[ix]) + p.pk_wmin;
+                    } else {
+                        vv[j] = x & p.pk_mask;
+                        ww[j] = (x >> p.pk_shift) + p.pk_wmin;
+                    }
                 } else {
                     EdgeT ed;
                     if (ZC) ed = p.zc[s_src[k] + (e - s_p[k])];
@@ -515,7 +521,7 @@ static int plan_dense_chunks(asc_graph* g, TravState* t) {
 
 int trav_ensure(asc_graph* g) {
     const uint64_t V = g->h.V;
-    const size_t EB = g->packed ? 4 : g->h.edge_bytes();  // bytes per streamed edge
+    const size_t EB = g->stream_record_bytes();  // bytes per streamed edge
     TravState* t = g->trav;
     const bool need_parts = g->static_vertices < V;
     if (t && t->V == V && (!need_parts || (t->od_list && t->part_edges >= g->part_edges))) return ASC_OK;
@@ -616,11 +622,14 @@ static uint32_t sssp_delta(asc_graph* g) {
 // concatenated on-demand edge list into `dst` (the reference's
 // fillEdgeArrByMultiThread, GraphMeta.cuh:469-514, cut to one part).
 static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long long* prefix, uint64_t m,
-                        uint64_t lo, uint64_t hi, char* dst) {
-    if (g->packed)
+                        uint64_t lo, uint64_t hi, char* dst, char* dst_w) {
+    if (g->packed) {
         gather_edges(*g->ctx->pool, (const char*)g->ptail.p, g->static_edges, g->h.off.p, 4, list, prefix, m, lo, hi,
                      dst);
-    else
+        if (g->pk_split)   // the weight bytes, at the same positions of their own array
+            gather_edges(*g->ctx->pool, (const char*)g->ptail_w.p, g->static_edges, g->h.off.p, 1, list, prefix, m,
+                         lo, hi, dst_w);
+    } else
         gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, 0, g->h.off.p, g->h.edge_bytes(), list, prefix, m, lo,
                      hi, dst);
 }
@@ -654,6 +663,34 @@ static void choose_dense_chunks(TravState* t, size_t EB) {
     }
 }
 
+// Copies `cnt` (at most one part) streamed records into device part buffer b:
+// whole records from `src`, or in the split form the ids from `src` and the
+// weight bytes from `src_w`, placed behind the buffer's part_edges ids.
+static int copy_to_part(asc_graph* g, TravState* t, int b, const char* src, const char* src_w, uint64_t cnt,
+                        hipStream_t s) {
+    if (cnt > t->part_edges) {
+        set_last_error("part copy of %llu records exceeds the part buffer", (unsigned long long)cnt);
+        return ASC_ERR_STATE;
+    }
+    if (g->pk_split) {
+        ASC_HIP(hipMemcpyAsync(t->part[b], src, 4 * cnt, hipMemcpyHostToDevice, s));
+        ASC_HIP(hipMemcpyAsync((char*)t->part[b] + 4 * t->part_edges, src_w, cnt, hipMemcpyHostToDevice, s));
+    } else {
+        ASC_HIP(hipMemcpyAsync(t->part[b], src, cnt * g->stream_record_bytes(), hipMemcpyHostToDevice, s));
+    }
+    return ASC_OK;
+}
+
+// Host addresses of on-demand edge e in the pinned tail: its record (the id
+// word in the packed forms) and, in the split form, its weight byte.
+static const char* tail_rec(const asc_graph* g, uint64_t e) {
+    const uint64_t i = e - g->static_edges;
+    return g->packed ? (const char*)(g->ptail.p + i) : (const char*)g->zc_tail.p + i * g->h.edge_bytes();
+}
+static const char* tail_w8(const asc_graph* g, uint64_t e) {
+    return g->pk_split ? (const char*)(g->ptail_w.p + (e - g->static_edges)) : nullptr;
+}
+
 // Streams one pile: filter on the device, list to the host, u64 prefix of
 // degrees, then parts (gather + double-buffered copies) or zero-copy reads.
 // With dense chunks planned, the pile's active edges are first counted per
@@ -670,17 +707,12 @@ static int prefetch_chunks(asc_graph* g, TravState* t) {
     for (int b = 0; b < kPartBufs; b++) t->pref_chunk[b] = -1;
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
     if (!n_ch || ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || ctx->cfg.dense_threshold > 1.0) return ASC_OK;
-    const bool pk = g->packed;
-    const char* tail = pk ? (const char*)g->ptail.p : (const char*)g->zc_tail.p;
-    if (!tail) return ASC_OK;
-    const size_t EB = pk ? 4 : g->h.edge_bytes();
-    const uint64_t S = g->static_edges;
+    if (!g->packed && !g->zc_tail.p) return ASC_OK;
     for (int b = 0; b < kPartBufs && (uint32_t)b < n_ch; b++) {
         const uint64_t e0 = t->ch_e[b], e1 = t->ch_e[b + 1];
         if (e1 - e0 > t->part_edges) break;
         ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
-        ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
-                               ctx->s_copy));
+        ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
         ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
         t->pref_chunk[b] = b;
     }
@@ -710,7 +742,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     const uint64_t nod = g->h.V - g->static_vertices;
     uint32_t* list_deg = t->od_list + nod;
     const bool pk = g->packed;
-    const size_t EB = pk ? 4 : g->h.edge_bytes();
+    const size_t EB = g->stream_record_bytes();
     const uint64_t S = g->static_edges;
     const int pile_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096));
 
@@ -823,8 +855,24 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     p.n = m;
     p.tail_base = S;
     p.pk_shift = g->pk_shift;
-    p.pk_mask = pk ? (1u << g->pk_shift) - 1 : 0xFFFFFFFFu;
+    p.pk_mask = pk && !g->pk_split ? (1u << g->pk_shift) - 1 : 0xFFFFFFFFu;
     p.pk_wmin = g->pk_wmin;
+    p.pk_split = g->pk_split ? 1u : 0u;
+    const uint64_t PE = t->part_edges;
+    // mapped device addresses of the pinned tail (zero-copy and hybrid reads)
+    auto map_tail = [&]() -> int {
+        void* dptr = nullptr;
+        ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
+        p.zc = (const EdgeT*)dptr;
+        p.zc32 = (const uint32_t*)dptr;
+        p.zc_w8 = nullptr;
+        if (g->pk_split) {
+            void* wptr = nullptr;
+            ASC_HIP(hipHostGetDevicePointer(&wptr, (void*)g->ptail_w.p, 0));
+            p.zc_w8 = (const uint8_t*)wptr;
+        }
+        return ASC_OK;
+    };
     auto launch_part = [&](bool zc) {
         const int pgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (p.hi - p.lo + kTile - 1) / kTile));
         return timed_launch(t, prof, KERN_PART, s, [&] {
@@ -863,10 +911,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     };
     uint64_t dense_edges = 0;
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
-        void* dptr = nullptr;
-        ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
-        p.zc = (const EdgeT*)dptr;
-        p.zc32 = (const uint32_t*)dptr;
+        ASC_TRY(map_tail());
         p.lo = 0;
         p.hi = T;
         ASC_TRY(launch_part(true));
@@ -876,21 +921,20 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         uint32_t nb = 0;   // device part buffers are used round-robin by both loops
         // dense chunks: whole-chunk DMA from the pinned tail, then the chunk's
         // list segment expanded with edges addressed from the chunk start
-        const char* tail = pk ? (const char*)g->ptail.p : (const char*)g->zc_tail.p;
         for (uint32_t c = 0; c < n_ch && n_dense; c++) {
             if (!t->h_ch_dense[c]) continue;
             const int b = (int)(nb++ % kPartBufs);
             const uint64_t e0 = t->ch_e[c], e1 = t->ch_e[c + 1];
             if (t->pref_chunk[b] != (int)c) {   // not copied ahead at the start of the run
                 ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
-                ASC_HIP(hipMemcpyAsync(t->part[b], tail + (e0 - S) * EB, (e1 - e0) * EB, hipMemcpyHostToDevice,
-                                       ctx->s_copy));
+                ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
                 ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
             }
             t->pref_chunk[b] = -1;
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.zc = (const EdgeT*)t->part[b];
             p.zc32 = (const uint32_t*)t->part[b];
+            p.zc_w8 = (const uint8_t*)t->part[b] + 4 * PE;
             p.tail_base = e0;
             p.lo = t->h_ch_lo[c];
             p.hi = t->h_ch_lo[c] + t->h_ch_edges[c];
@@ -902,10 +946,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         if (ctx->cfg.stream_mode == ASC_STREAM_HYBRID) {
             // the rest read by the kernel from the mapped pinned tail
             if (Ts > 0) {
-                void* dptr = nullptr;
-                ASC_HIP(hipHostGetDevicePointer(&dptr, pk ? (void*)g->ptail.p : (void*)g->zc_tail.p, 0));
-                p.zc = (const EdgeT*)dptr;
-                p.zc32 = (const uint32_t*)dptr;
+                ASC_TRY(map_tail());
                 p.tail_base = S;
                 p.lo = 0;
                 p.hi = Ts;
@@ -920,7 +961,6 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         }
         // the rest: host gather into pinned staging, double-buffered copies
         double fill_round = 0;
-        const uint64_t PE = t->part_edges;
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
             const int sb = (int)(q & 1);
@@ -928,15 +968,17 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             const uint64_t lo = q * PE, hi = std::min(Ts, lo + PE);
             if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
-            gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p);
+            char* stage_w = t->stage[sb].p + 4 * PE;   // split form: weight bytes behind the ids
+            gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p, stage_w);
             fill_round += now_ms() - tg;
             ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
-            ASC_HIP(hipMemcpyAsync(t->part[b], t->stage[sb].p, (hi - lo) * EB, hipMemcpyHostToDevice, ctx->s_copy));
+            ASC_TRY(copy_to_part(g, t, b, t->stage[sb].p, stage_w, hi - lo, ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_stage[sb], ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.part = (const EdgeT*)t->part[b];
             p.part32 = (const uint32_t*)t->part[b];
+            p.part_w8 = (const uint8_t*)t->part[b] + 4 * PE;
             p.tail_base = S;
             p.lo = lo;
             p.hi = hi;
@@ -1172,7 +1214,7 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
         st->static_edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_bu + ds.edges_uf;
         st->edges_traversed = st->static_edges_traversed + ds.edges_part;
         st->streamed_edges = rc.streamed_edges;
-        st->streamed_bytes = rc.streamed_edges * (g->packed ? 4 : g->h.edge_bytes());
+        st->streamed_bytes = rc.streamed_edges * g->stream_record_bytes();
         st->dense_edges = rc.dense_edges;
         st->vertices_processed = ds.verts_local + ds.verts_bu;
         st->host_cores = hardware_threads();

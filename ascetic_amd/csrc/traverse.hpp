@@ -98,8 +98,11 @@ struct PartArgs {
     const EdgeT* zc;                // zero-copy: mapped host edges from tail_base
     const uint32_t* part32;         // packed stream: device buffer of u32 words
     const uint32_t* zc32;           // packed stream, zero-copy
+    const uint8_t* part_w8;         // split stream: weight bytes beside part32
+    const uint8_t* zc_w8;           // split stream: weight bytes beside zc32
     unsigned long long tail_base;
     uint32_t pk_shift, pk_mask, pk_wmin;  // word = dst | (w - pk_wmin) << pk_shift
+    uint32_t pk_split;              // 1: word = dst, weight byte = w - pk_wmin
 };
 
 struct RunCounters {

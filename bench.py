@@ -279,7 +279,7 @@ def stream_section(A, _abi, args):
     budget = int(args.budget_frac * E * s_e)
     mode = {"gather": _abi.STREAM_GATHER, "zerocopy": _abi.STREAM_ZEROCOPY, "hybrid": _abi.STREAM_HYBRID}[args.stream_mode]
     ctx = A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, stream_mode=mode, profile=True,
-                    sssp_delta=args.delta)
+                    sssp_delta=args.delta, stream_pack=args.stream_pack)
     g = ctx.from_arrays(off, ew)
     info = g.info()
     log(f"plan: static edges {info['static_edges']} ({info['static_edges'] / E:.1%}), static vertices "
@@ -435,6 +435,9 @@ def main():
     ap.add_argument("--budget-frac", type=float, default=0.4)
     ap.add_argument("--part-mb", type=int, default=64)
     ap.add_argument("--stream-mode", choices=["gather", "zerocopy", "hybrid"], default="gather")
+    ap.add_argument("--stream-pack", type=int, default=1, choices=[0, 1, 2],
+                    help="streamed record form: 1 packed (4 bytes, else 5 when the weights fit a byte), "
+                         "2 always 5 bytes, 0 8-byte {dst, w} records")
     ap.add_argument("--delta", type=int, default=0, help="SSSP band width (0 = library default, <0 = off)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stream-baselines", dest="stream_baselines", action="store_false",

@@ -138,7 +138,11 @@ typedef struct {
                                     on its launch stream (asc_stats.kern)    */
     int stream_pack;             /* 1 (default): stream weighted on-demand
                                     edges as one u32 word, dst | (w - wmin) <<
-                                    bits(V-1), when both fit; 0: 8-byte records */
+                                    bits(V-1), when both fit, else as a u32 id
+                                    plus a byte w - wmin when the weights span
+                                    at most 256 values (5 bytes); 2: that
+                                    split form whenever the weights allow it;
+                                    0: 8-byte records                        */
     int bfs_direction;           /* asc_bfs_direction                         */
     int cc_method;               /* asc_cc_method                             */
     double dense_threshold;      /* gather streaming: an on-demand chunk (a

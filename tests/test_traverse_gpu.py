@@ -91,15 +91,16 @@ def test_streaming_matches_resident(name, mk, mode):
             lab, st = g.cc()
             assert np.array_equal(lab, O.cc_minreach(csr))
             assert st["streamed_edges"] > 0
-    for pack in (True, False):  # 4-byte packed stream and 8-byte {dst, w} records
+    # stream_pack 1: 4-byte words; 2: u32 ids plus weight bytes; 0: 8-byte {dst, w} records
+    for pack, rec in ((1, 4), (2, 5), (0, 8)):
         with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_mode=mode,
                        stream_pack=pack) as ctx:
             with make_graph(ctx, off, dst, w) as g:
                 for s in (0, int(np.argmax(np.diff(off)))):
                     d, st = g.sssp(s)
-                    assert np.array_equal(d, O.dijkstra(csr, s))
+                    assert np.array_equal(d, O.dijkstra(csr, s)), (pack, s)
                     if st["streamed_edges"]:
-                        assert st["streamed_bytes"] == st["streamed_edges"] * (4 if pack else 8)
+                        assert st["streamed_bytes"] == st["streamed_edges"] * rec
                     lv, _ = g.bfs(s)
                     assert np.array_equal(lv, O.bfs_queue(csr, s))
 
@@ -109,7 +110,7 @@ def test_streaming_matches_resident(name, mk, mode):
 def test_dense_chunk_copies(name, mk, thr):
     """On-demand chunks copied whole by DMA (threshold ~0: every active chunk;
     0.5: the default; 2: never) give the same values as the oracle, in the
-    4-byte packed, 8-byte and unweighted layouts."""
+    4-byte packed, split 5-byte, 8-byte and unweighted layouts."""
     say("dense", name, "threshold", thr)
     off, dst, _ = mk()
     w = weights_for(off, 6)
@@ -127,14 +128,14 @@ def test_dense_chunk_copies(name, mk, thr):
             lab, st = g.cc()
             assert np.array_equal(lab, O.cc_minreach(csr))
             dense += st["dense_edges"]
-    for pack in (True, False):
+    for pack, rec in ((1, 4), (2, 5), (0, 8)):
         with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_pack=pack,
                        dense_threshold=thr) as ctx:
             with make_graph(ctx, off, dst, w) as g:
                 for s in srcs:
                     d, st = g.sssp(s)
                     assert np.array_equal(d, O.dijkstra(csr, s)), (pack, s)
-                    assert st["streamed_bytes"] == st["streamed_edges"] * (4 if pack else 8)
+                    assert st["streamed_bytes"] == st["streamed_edges"] * rec
                     dense += st["dense_edges"]
     if thr > 1:
         assert dense == 0
@@ -142,18 +143,20 @@ def test_dense_chunk_copies(name, mk, thr):
         assert dense > 0
 
 
+@pytest.mark.parametrize("pack", [1, 2])
 @pytest.mark.parametrize("thr", [1e-9, 2.0])
 @pytest.mark.parametrize("name,mk", CASES[3:])
-def test_stream_overlap_exact(name, mk, thr):
+def test_stream_overlap_exact(name, mk, thr, pack):
     """Resident steps between streamed chunks / parts (SSSP, CC) give the
-    oracle's values, with whole-chunk copies and with gathering only."""
-    say("overlap", name, "threshold", thr)
+    oracle's values, with whole-chunk copies and with gathering only, in the
+    4-byte and the split 5-byte stream forms."""
+    say("overlap", name, "threshold", thr, "pack", pack)
     off, dst, _ = mk()
     w = weights_for(off, 8)
     csr = O.CSR(off, dst, w)
     E = int(off[-1])
     with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 2048, dense_threshold=thr,
-                   stream_overlap=True) as ctx:
+                   stream_overlap=True, stream_pack=pack) as ctx:
         with make_graph(ctx, off, dst, w) as g:
             for s in (0, int(np.argmax(np.diff(off))), len(off) - 2):
                 d, st = g.sssp(s)
@@ -164,15 +167,16 @@ def test_stream_overlap_exact(name, mk, thr):
             assert np.array_equal(lv, O.bfs_queue(csr, 0)), name
 
 
+@pytest.mark.parametrize("pack", [1, 2])
 @pytest.mark.parametrize("mode", [_abi.STREAM_GATHER, _abi.STREAM_ZEROCOPY])
-def test_everything_on_demand(gpu_ctx, mode):
+def test_everything_on_demand(gpu_ctx, mode, pack):
     """A device budget of one byte: every edge is on demand. In zero-copy mode
     kernels read the whole graph from host memory at the link, the form the
     reference's unified-memory ("Share") drivers take without page migration."""
     off, dst, _ = G.random_graph(3000, 12000, 9, symmetric=False)
     w = weights_for(off, 9)
     csr = O.CSR(off, dst, w)
-    with A.Context(edge_budget_bytes=1, part_bytes=8 * 1024, stream_mode=mode) as ctx:
+    with A.Context(edge_budget_bytes=1, part_bytes=8 * 1024, stream_mode=mode, stream_pack=pack) as ctx:
         with make_graph(ctx, off, dst, w) as g:
             assert g.info()["static_vertices"] <= 1
             d, st = g.sssp(5)
