@@ -253,8 +253,9 @@ def test_serving_errors(gpu_ctx):
                 srv.submit(A.Server.BFS, 1 << 20)
             assert e.value.code == _abi.ERR_INVALID
             # result buffers the library could overrun are refused before submission
-            for bad in (np.empty(h.num_vertices - 1, np.uint32), np.empty(h.num_vertices, np.int64),
-                        np.empty(2 * h.num_vertices, np.uint32)[::2]):
+            # (adopt() took the host graph over, so V comes from the device graph)
+            V = g.num_vertices
+            for bad in (np.empty(V - 1, np.uint32), np.empty(V, np.int64), np.empty(2 * V, np.uint32)[::2]):
                 with pytest.raises(ValueError):
                     srv.submit(A.Server.BFS, 0, bad)
             srv.drain()
