@@ -194,11 +194,15 @@ def split_parts(prefix, part_edges):
 
 
 class Context:
-    """One HIP device plus the library's streams and gather threads."""
+    """One HIP device plus the library's streams and gather threads.  The
+    keyword arguments are the fields of asc_config (include/ascetic.h); e.g.
+    stream_pack selects the streamed record form: 1 packed (4 bytes when id and
+    weight share a word, else 5: a u32 id plus a weight byte), 2 always the
+    5-byte form, 0 the 8-byte {dst, w} records."""
 
     def __init__(self, device=0, advise_rate=0.0, edge_budget_bytes=0, part_bytes=0, device_mem_limit=0,
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
-                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=True,
+                 heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=1,
                  bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO, dense_threshold=0.0, stream_overlap=None):
         lib = _abi.load()
         cfg = _abi.Config()
