@@ -64,6 +64,7 @@ void usage() {
             "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy|hybrid] [--format fmt]\n"
             "               [--values ref|zero] [--pr-mode ref|norm] [--pr-method auto|pull|blocked] [--iters n]\n"
             "               [--out file] [--verbose 0|1|2] [--dense f] [--overlap 0|1] [--bfs td|auto] [--cc lp|auto]\n"
+            "               [--pack 0|1|2]  (streamed records: 8-byte / packed 4 or 5 bytes / always 5 bytes)\n"
             "       ptgraph --type gen --gen rmat|er [--scale s --ef f | --vertices n --samples m] [--seed x]\n"
             "               [--weighted 0|1] [--wmax w] [--csc 0|1] --format fmt --out file\n"
             "       ptgraph --type convert --input edges.txt|.mtx|.toB --format fmt --out file [--symmetrize 0|1]\n"
@@ -202,6 +203,7 @@ int main(int argc, char** argv) {
     cfg.stream_mode = sm == "zerocopy" ? ASC_STREAM_ZEROCOPY : sm == "hybrid" ? ASC_STREAM_HYBRID : ASC_STREAM_GATHER;
     cfg.dense_threshold = a.num("--dense", 0.0);
     cfg.stream_overlap = (int)a.num("--overlap", 1);
+    cfg.stream_pack = (int)a.num("--pack", 1);
     cfg.bfs_direction = a.get("--bfs", "auto") == "td" ? ASC_BFS_TOP_DOWN : ASC_BFS_AUTO;
     cfg.cc_method = a.get("--cc", "auto") == "lp" ? ASC_CC_LABEL_PROP : ASC_CC_AUTO;
     cfg.value_convention = a.get("--values", "ref") == "zero" ? ASC_VALUES_ZERO_BASED : ASC_VALUES_REFERENCE;
