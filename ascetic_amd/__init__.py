@@ -286,8 +286,10 @@ class Graph:
         self._h = h
 
     def close(self):
+        """Frees the graph; refused (AscError, ERR_STATE) while a server over
+        it is open, and the handle then stays valid."""
         if self._h:
-            _abi.load().asc_graph_destroy(self._h)
+            check(_abi.load().asc_graph_destroy(self._h), "asc_graph_destroy")
             self._h = None
 
     def __enter__(self):
@@ -461,4 +463,4 @@ def cc_opt(path, advise_rate=0.0, verbose=True):
 def pr_opt(path, advise_rate=0.0, verbose=True, mode=_abi.PR_REFERENCE):
     """PageRank of a BCSC file; by default in the reference's FP64 active-set
     form (0.15 + 0.85 * sum, threshold 0.001)."""
-    return _driver(path, advise_rate, _abi.FMT_AUTO, verbose, lambda g: g.pagerank(mode=mode))
+    return _driver(path, advise_rate, _abi.FMT_BCSC, verbose, lambda g: g.pagerank(mode=mode))

@@ -28,7 +28,7 @@ KIND_CSR, KIND_WCSR, KIND_CSC = range(3)
 VALUES_REFERENCE, VALUES_ZERO_BASED = 0, 1
 STREAM_GATHER, STREAM_ZEROCOPY, STREAM_HYBRID = 0, 1, 2
 PR_NORMALISED, PR_REFERENCE = 0, 1
-PR_AUTO, PR_PULL, PR_BLOCKED = 0, 1, 2
+PR_AUTO, PR_PULL, PR_BLOCKED, PR_TILED = 0, 1, 2, 3
 BFS_AUTO, BFS_TOP_DOWN = 0, 1
 CC_AUTO, CC_LABEL_PROP = 0, 1
 QUERY_BFS, QUERY_SSSP = 0, 1

@@ -103,6 +103,7 @@ int asc_tuning_apply(asc_graph* g, const asc_tuning* t) {
 
 int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out) {
     if (!g) return ASC_ERR_INVALID;
+    ASC_TRY(graph_ready(g));
     if (budget_ms <= 0) budget_ms = 30000;
     const double t_start = now_ms();
     auto over = [&] { return now_ms() - t_start > budget_ms; };

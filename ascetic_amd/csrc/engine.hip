@@ -39,18 +39,39 @@ static int param_size_for(int kind) {
     return kind == ASC_KIND_WCSR ? 15 : kind == ASC_KIND_CSC ? 17 : 13;
 }
 
+int graph_ready(const asc_graph* g) {
+    if (g->usable) return ASC_OK;
+    set_last_error("the graph lost its device copies in a failed re-plan; re-plan it again");
+    return ASC_ERR_STATE;
+}
+
+// Plans the split with the graph's budget and uploads the static prefix.  The
+// plan and its device size are checked before anything is freed, so a plan
+// that cannot fit (asc_config.device_mem_limit caps the device bytes of one
+// graph) leaves the current copies in place; a HIP allocation or copy that
+// fails after the old copies were freed leaves the graph unusable.
 int graph_upload(asc_graph* g) {
     asc_ctx* ctx = g->ctx;
     ASC_HIP(hipSetDevice(ctx->device));
-    graph_free_device(g);
     const uint64_t V = g->h.V, E = g->h.E;
     const size_t EB = g->h.edge_bytes();
     size_t free_b = 0, total_b = 0;
     ASC_HIP(hipMemGetInfo(&free_b, &total_b));
+    // the graph's current device copies return to the pool when it is re-planned
+    free_b += g->device_bytes;
     if (ctx->cfg.device_mem_limit) free_b = std::min<size_t>(free_b, ctx->cfg.device_mem_limit);
     asc_plan plan;
-    ASC_TRY(asc_plan_partition(V, E, g->h.off.p, EB, free_b, param_size_for(g->h.kind), ctx->cfg.advise_rate,
-                               ctx->cfg.edge_budget_bytes, ctx->cfg.part_bytes, &plan));
+    ASC_TRY(asc_plan_partition(V, E, g->h.off.p, EB, free_b, param_size_for(g->h.kind), g->advise_rate,
+                               g->edge_budget_bytes, ctx->cfg.part_bytes, &plan));
+    const uint64_t need = (uint64_t)g->off_bytes * (V + 1) + std::max<uint64_t>(EB * plan.static_edges, 64) +
+                          (g->h.kind == ASC_KIND_CSC ? 4 * V : 0);
+    if (ctx->cfg.device_mem_limit && need > ctx->cfg.device_mem_limit) {
+        set_last_error("the planned device copies need %llu bytes, above the %llu-byte device limit",
+                       (unsigned long long)need, (unsigned long long)ctx->cfg.device_mem_limit);
+        return ASC_ERR_NOMEM;
+    }
+    graph_free_device(g);
+    g->usable = false;
     g->static_edges = plan.static_edges;
     g->static_vertices = plan.static_vertices;
     g->part_edges = plan.part_edges;
@@ -157,6 +178,7 @@ int graph_upload(asc_graph* g) {
     // union-find CC); checked on the device copy when every edge is on it
     g->symmetric = -1;
     if (g->static_vertices == V) ASC_TRY(check_symmetric(g));
+    g->usable = true;
     return ASC_OK;
 }
 
@@ -175,6 +197,8 @@ static void compute_max_degree(asc_graph* g) {
 
 static int finish_graph(asc_ctx* ctx, asc_graph* g, asc_graph** out) {
     g->ctx = ctx;
+    g->edge_budget_bytes = ctx->cfg.edge_budget_bytes;
+    g->advise_rate = ctx->cfg.advise_rate;
     if (g->h.E >= (uint64_t(1) << 32)) g->off_bytes = 8;
     g->heavy_threshold = ctx->cfg.heavy_threshold;
     g->pr_group = ctx->cfg.pr_group;
@@ -364,6 +388,10 @@ int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph**
 
 int asc_graph_destroy(asc_graph* g) {
     if (!g) return ASC_ERR_INVALID;
+    if (g->servers.load() > 0) {
+        set_last_error("a server still runs over this graph; destroy it first");
+        return ASC_ERR_STATE;
+    }
     (void)hipSetDevice(g->ctx->device);
     graph_free_device(g);
     delete g;
@@ -390,13 +418,25 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info) {
 
 int asc_graph_replan(asc_graph* g, uint64_t edge_budget_bytes, double advise_rate) {
     if (!g) return ASC_ERR_INVALID;
-    g->ctx->cfg.edge_budget_bytes = edge_budget_bytes;
-    g->ctx->cfg.advise_rate = advise_rate;
-    return graph_upload(g);
+    if (g->servers.load() > 0) {
+        set_last_error("a server still runs over this graph; destroy it before re-planning");
+        return ASC_ERR_STATE;
+    }
+    const uint64_t old_budget = g->edge_budget_bytes;
+    const double old_rate = g->advise_rate;
+    g->edge_budget_bytes = edge_budget_bytes;
+    g->advise_rate = advise_rate;
+    const int r = graph_upload(g);
+    if (r != ASC_OK && g->usable) {   // refused before anything changed: keep the old plan's inputs
+        g->edge_budget_bytes = old_budget;
+        g->advise_rate = old_rate;
+    }
+    return r;
 }
 
 int asc_bfs(asc_graph* g, uint32_t source, uint32_t* level_out, asc_stats* st) {
     if (!g) return ASC_ERR_INVALID;
+    ASC_TRY(graph_ready(g));
     if (source >= g->h.V) {
         set_last_error("source %u >= V %llu", source, (unsigned long long)g->h.V);
         return ASC_ERR_INVALID;
@@ -407,6 +447,7 @@ int asc_bfs(asc_graph* g, uint32_t source, uint32_t* level_out, asc_stats* st) {
 
 int asc_sssp(asc_graph* g, uint32_t source, uint32_t* dist_out, asc_stats* st) {
     if (!g) return ASC_ERR_INVALID;
+    ASC_TRY(graph_ready(g));
     if (source >= g->h.V) {
         set_last_error("source %u >= V %llu", source, (unsigned long long)g->h.V);
         return ASC_ERR_INVALID;
@@ -420,6 +461,7 @@ int asc_sssp(asc_graph* g, uint32_t source, uint32_t* dist_out, asc_stats* st) {
 
 int asc_cc(asc_graph* g, uint32_t* label_out, asc_stats* st) {
     if (!g) return ASC_ERR_INVALID;
+    ASC_TRY(graph_ready(g));
     if (g->h.kind == ASC_KIND_CSC) return ASC_ERR_UNSUPPORTED;
     return traverse(g, ALG_CC, 0, label_out, st);
 }
@@ -427,6 +469,7 @@ int asc_cc(asc_graph* g, uint32_t* label_out, asc_stats* st) {
 int asc_pr(asc_graph* g, const asc_pr_opts* opts, float* rank_f32, double* rank_f64, double* residual_out,
            asc_stats* st) {
     if (!g) return ASC_ERR_INVALID;
+    ASC_TRY(graph_ready(g));
     asc_pr_opts o;
     if (opts) o = *opts; else asc_pr_opts_init(&o);
     if (o.mode != ASC_PR_NORMALISED && o.mode != ASC_PR_REFERENCE) return ASC_ERR_INVALID;

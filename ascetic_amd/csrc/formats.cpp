@@ -106,6 +106,10 @@ int validate_csr(const HostCSR& h) {
     return ASC_OK;
 }
 
+// Reads an open graph file as layout `fmt` (header already parsed).
+static int read_graph_as(FILE* f, const char* path, uint64_t size, int fmt, uint64_t V64, uint64_t E64,
+                         uint32_t V32, uint32_t E32, HostCSR& out);
+
 int read_graph_file(const char* path, int format, HostCSR& out) {
     if (!path) return ASC_ERR_INVALID;
     struct stat sb;
@@ -134,23 +138,33 @@ int read_graph_file(const char* path, int format, HostCSR& out) {
         std::memcpy(&V64, hdr, 8);
         std::memcpy(&E64, hdr + 8, 8);
     }
-    int fmt = format;
-    if (fmt == ASC_FMT_AUTO) {
-        const int order[] = {ASC_FMT_BCSR, ASC_FMT_BWCSR, ASC_FMT_BCSC, ASC_FMT_LEGACY32, ASC_FMT_LEGACY32W};
-        for (int f : order) {
-            bool wide = f <= ASC_FMT_BCSC;
-            uint64_t V = wide ? V64 : V32, E = wide ? E64 : E32;
-            if (wide && got < 16) continue;
-            if (header_plausible(V, E, size) && layout_size(f, V, E) == size) {
-                fmt = f;
-                break;
-            }
+    if (format != ASC_FMT_AUTO) return read_graph_as(fh.f, path, size, format, V64, E64, V32, E32, out);
+    // Every layout whose size matches the header is tried in this order; the
+    // first that reads and validates wins.  Two layouts can match one size
+    // (BWCSR and BCSC when V == E), so a size match alone does not decide.
+    const int order[] = {ASC_FMT_BCSR, ASC_FMT_BWCSR, ASC_FMT_BCSC, ASC_FMT_LEGACY32, ASC_FMT_LEGACY32W};
+    int r = ASC_ERR_FORMAT;
+    bool matched = false;
+    for (int f : order) {
+        const bool wide = f <= ASC_FMT_BCSC;
+        const uint64_t V = wide ? V64 : V32, E = wide ? E64 : E32;
+        if (wide && got < 16) continue;
+        if (!header_plausible(V, E, size) || layout_size(f, V, E) != size) continue;
+        matched = true;
+        HostCSR h;
+        r = read_graph_as(fh.f, path, size, f, V64, E64, V32, E32, h);
+        if (r == ASC_OK) {
+            out = std::move(h);
+            return ASC_OK;
         }
-        if (fmt == ASC_FMT_AUTO) {
-            set_last_error("%s: size %llu matches no layout", path, (unsigned long long)size);
-            return ASC_ERR_FORMAT;
-        }
+        if (r != ASC_ERR_FORMAT && r != ASC_ERR_RANGE) return r;   // I/O or memory: no other layout helps
     }
+    if (!matched) set_last_error("%s: size %llu matches no layout", path, (unsigned long long)size);
+    return r;
+}
+
+static int read_graph_as(FILE* f, const char* path, uint64_t size, int fmt, uint64_t V64, uint64_t E64,
+                         uint32_t V32, uint32_t E32, HostCSR& out) {
     const bool wide = fmt == ASC_FMT_BCSR || fmt == ASC_FMT_BWCSR || fmt == ASC_FMT_BCSC;
     const uint64_t V = wide ? V64 : V32, E = wide ? E64 : E32;
     if (fmt < ASC_FMT_BCSR || fmt > ASC_FMT_LEGACY32W) return ASC_ERR_INVALID;
@@ -159,7 +173,7 @@ int read_graph_file(const char* path, int format, HostCSR& out) {
                        (unsigned long long)V, (unsigned long long)E);
         return ASC_ERR_FORMAT;
     }
-    if (fseek(fh.f, wide ? 16 : 8, SEEK_SET) != 0) return ASC_ERR_IO;
+    if (fseek(f, wide ? 16 : 8, SEEK_SET) != 0) return ASC_ERR_IO;
     out.V = V;
     out.E = E;
     out.offset_bytes = wide ? 8 : 4;
@@ -168,22 +182,22 @@ int read_graph_file(const char* path, int format, HostCSR& out) {
                                          : ASC_KIND_CSR;
     if (fmt == ASC_FMT_BCSC) {
         ASC_TRY(out.outdeg.alloc(V));
-        ASC_TRY(read_exact(fh.f, out.outdeg.p, 4 * V));
+        ASC_TRY(read_exact(f, out.outdeg.p, 4 * V));
     }
     ASC_TRY(out.off.alloc(V + 1));
     if (wide) {
-        ASC_TRY(read_exact(fh.f, out.off.p, 8 * V));
+        ASC_TRY(read_exact(f, out.off.p, 8 * V));
     } else {
         HostBuf<uint32_t> o32;
         ASC_TRY(o32.alloc(V));
-        ASC_TRY(read_exact(fh.f, o32.p, 4 * V));
+        ASC_TRY(read_exact(f, o32.p, 4 * V));
         host_pool().parallel_for(V, [&](uint64_t b, uint64_t e) {
             for (uint64_t i = b; i < e; i++) out.off[i] = o32[i];
         });
     }
     out.off[V] = E;
     ASC_TRY(out.edges.alloc(E * out.edge_words() + 1));
-    int r = read_exact(fh.f, out.edges.p, E * out.edge_bytes());
+    int r = read_exact(f, out.edges.p, E * out.edge_bytes());
     if (r != ASC_OK) {
         set_last_error("%s: short read of the edge array", path);
         return r;

@@ -195,6 +195,17 @@ struct asc_graph {
     asc_ctx* ctx = nullptr;
     asc::HostCSR h;
     int off_bytes = 8;              // device offset width
+    // sizing inputs of this graph's plan (from the context's config at
+    // creation; asc_graph_replan changes them for this graph only)
+    uint64_t edge_budget_bytes = 0;
+    double advise_rate = 0;
+    // false after a re-plan whose upload failed part-way: the device copies
+    // are gone and every algorithm call returns ASC_ERR_STATE until a
+    // successful asc_graph_replan
+    bool usable = true;
+    // servers created over this graph and not yet destroyed; re-planning or
+    // destroying the graph is refused while any exists
+    std::atomic<int> servers{0};
     // static / on-demand plan
     uint64_t static_edges = 0;      // edges [0, static_edges) resident on device
     uint64_t static_vertices = 0;   // vertices [0, static_vertices) are static
@@ -238,6 +249,7 @@ struct asc_graph {
 
 namespace asc {
 int graph_upload(asc_graph* g);       // device copies after a (re)plan
+int graph_ready(const asc_graph* g);  // ASC_OK, or ASC_ERR_STATE for an unusable graph
 void graph_free_device(asc_graph* g);
 void trav_free(TravState* t);
 void pr_free(PrState* p);

@@ -62,6 +62,16 @@ struct PrState {
     float* pb_vals = nullptr;
     unsigned long long* pb_bin_start = nullptr;   // region bases [b * R + r]
     uint32_t* pb_cnt = nullptr;                   // region sizes [b * R + r]
+    // source-segmented normalised mode (every edge resident): the light
+    // in-edges regrouped into tiles (destination block d, source segment s),
+    // tile (d, s) at tl_edges[tl_off[d * tl_ns + s] ...], one u32 word per
+    // edge (block-local destination << kTlSrcBits | segment-local source)
+    int tl_state = 0;             // 0 not built, 1 built, -1 not possible
+    uint32_t tl_nd = 0, tl_ns = 0;
+    uint64_t tl_edges_n = 0;
+    uint32_t* tl_edges = nullptr;
+    unsigned long long* tl_off = nullptr;
+    uint32_t* tl_heavy = nullptr; // bitmap of destinations left to k_pr_pull_heavy
 };
 
 void pr_free(PrState* p) {
@@ -73,6 +83,7 @@ void pr_free(PrState* p) {
     f(p->acc); f(p->heavy); f(p->val64); f(p->sum64); f(p->contrib64); f(p->active); f(p->count);
     f(p->part[0]); f(p->part[1]);
     f(p->pb_src); f(p->pb_pos); f(p->pb_dl); f(p->pb_vals); f(p->pb_bin_start); f(p->pb_cnt);
+    f(p->tl_edges); f(p->tl_off); f(p->tl_heavy);
     for (int i = 0; i < 2; i++) {
         if (p->ev_copy[i]) (void)hipEventDestroy(p->ev_copy[i]);
         if (p->ev_kern[i]) (void)hipEventDestroy(p->ev_kern[i]);
@@ -409,6 +420,130 @@ __global__ __launch_bounds__(kPbBlock) void k_pb_gather(PrArgs a, const unsigned
     }
 }
 
+// ---- source-segmented normalised mode --------------------------------------
+// The pull gathers one random 4-byte contribution per in-edge from an array
+// of 4V bytes (256 MB at V = 2^26, beyond the 4 MB L2 of an XCD and as large
+// as the MALL), so almost every gather moves a whole line from HBM.  Here the
+// light in-edges are regrouped once into tiles by (destination block of
+// 2^14 vertices, source segment of 2^18 vertices).  One workgroup owns a
+// destination block: it keeps the block's sums in LDS (FP32, 64 KB) and walks
+// the segments in order, so the gathers of a tile read one 1 MB slice of the
+// contributions, and the workgroups resident on an XCD, all walking the
+// segments in the same order, keep that slice in the XCD's L2.  A tile edge
+// is one u32 word (block-local destination << 18 | segment-local source):
+// 4 streamed bytes per edge instead of a 4-byte id plus a line per gather.
+// The last pass over the block finishes its vertices (rank, contribution,
+// residual, dangling mass) as the pull does.  Destinations with in-degree >=
+// the heavy threshold keep the FP64 pull (k_pr_pull_heavy).
+constexpr int kTlDstBits = 14;
+constexpr int kTlSrcBits = 18;
+constexpr uint32_t kTlDst = 1u << kTlDstBits;              // destinations per block
+constexpr uint32_t kTlSrcMask = (1u << kTlSrcBits) - 1;    // sources per segment - 1
+constexpr uint32_t kTlMaxSeg = 8192;                       // build histograms in LDS: V <= 2^31
+constexpr int kTlBlock = 1024;                             // workgroup of the tiled pull
+
+// cnt[d * ns + s] = light in-edges of destination block d from segment s.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_tl_count(const OffT* off, const uint32_t* src, uint64_t V,
+                                                     uint32_t heavy_thr, uint32_t ns, uint32_t* cnt) {
+    __shared__ uint32_t h[kTlMaxSeg];
+    const uint64_t d = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const uint64_t v0 = d << kTlDstBits, v1 = min<uint64_t>(V, v0 + kTlDst);
+    for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
+        const OffT b = off[v], e = off[v + 1];
+        if ((uint64_t)(e - b) >= heavy_thr) continue;
+        for (OffT k = b; k < e; k++) atomicAdd(&h[src[k] >> kTlSrcBits], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) cnt[d * ns + i] = h[i];
+}
+
+// Writes the tile words of destination block d (positions within a tile in
+// arrival order) and marks its heavy destinations in `heavy_bits`.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_tl_fill(const OffT* off, const uint32_t* src, uint64_t V,
+                                                    uint32_t heavy_thr, uint32_t ns, const unsigned long long* toff,
+                                                    uint32_t* tiles, uint32_t* heavy_bits) {
+    __shared__ uint32_t cur[kTlMaxSeg];
+    const uint64_t d = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) cur[i] = 0;
+    __syncthreads();
+    const uint64_t v0 = d << kTlDstBits, v1 = min<uint64_t>(V, v0 + kTlDst);
+    const unsigned long long* t = toff + d * ns;
+    for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
+        const OffT b = off[v], e = off[v + 1];
+        if ((uint64_t)(e - b) >= heavy_thr) {
+            atomicOr(&heavy_bits[v >> 5], 1u << (v & 31));
+            continue;
+        }
+        const uint32_t dl = (uint32_t)(v - v0) << kTlSrcBits;
+        for (OffT k = b; k < e; k++) {
+            const uint32_t u = src[k];
+            const uint32_t s = u >> kTlSrcBits;
+            tiles[t[s] + atomicAdd(&cur[s], 1u)] = dl | (u & kTlSrcMask);
+        }
+    }
+}
+
+// One workgroup per destination block; grid = number of blocks.
+__global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t* __restrict__ tiles,
+                                                       const unsigned long long* __restrict__ toff, uint32_t ns,
+                                                       const uint32_t* heavy_bits) {
+    __shared__ float acc[kTlDst];
+    __shared__ double s_r[kTlBlock / kWave], s_d[kTlBlock / kWave];
+    const uint64_t d = blockIdx.x;
+    for (uint32_t i = threadIdx.x; i < kTlDst; i += kTlBlock) acc[i] = 0.0f;
+    __syncthreads();
+    const unsigned long long* t = toff + d * ns;
+    unsigned long long k0 = t[0];
+    for (uint32_t s = 0; s < ns; s++) {
+        const unsigned long long k1 = t[s + 1];
+        const float* cseg = a.contrib_in + ((uint64_t)s << kTlSrcBits);
+        // four words per lane per pass: the words, then their gathers, then
+        // the LDS adds (lanes past the tile's end re-read its first word)
+        for (unsigned long long k = k0 + threadIdx.x; k < k1; k += 4 * kTlBlock) {
+            uint32_t w[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[j] = tiles[k + j * kTlBlock < k1 ? k + j * kTlBlock : k0];
+            float c[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) c[j] = cseg[w[j] & kTlSrcMask];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (k + j * kTlBlock < k1) atomicAdd(&acc[w[j] >> kTlSrcBits], c[j]);
+        }
+        k0 = k1;
+    }
+    __syncthreads();
+    const double base = (1.0 - a.damping64) / (double)a.V + a.damping64 * (*a.dang_in) / (double)a.V;
+    double res = 0, dang = 0;
+    const uint64_t v0 = d << kTlDstBits;
+    for (uint32_t i = threadIdx.x; i < kTlDst; i += kTlBlock) {
+        const uint64_t v = v0 + i;
+        if (v >= a.V) break;
+        if (heavy_bits && (heavy_bits[v >> 5] >> (v & 31) & 1u)) continue;
+        pr_finish_vertex(a, v, (double)acc[i], base, res, dang);
+    }
+    res = wave_sum(res);
+    dang = wave_sum(dang);
+    if (lane_id() == 0) {
+        s_r[threadIdx.x / kWave] = res;
+        s_d[threadIdx.x / kWave] = dang;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0, dd = 0;
+        for (int i = 0; i < kTlBlock / kWave; i++) {
+            r += s_r[i];
+            dd += s_d[i];
+        }
+        if (r != 0) atomicAdd(a.residual, r);
+        if (dd != 0) atomicAdd(a.dang_out, dd);
+    }
+}
+
 // ---- reference mode --------------------------------------------------------
 
 __global__ void k_prref_init(double* val, uint8_t* active, uint64_t V) {
@@ -611,6 +746,55 @@ static int pr_blocked_build(asc_graph* g, PrState* p) {
     return ASC_OK;
 }
 
+// Builds the tiles of the source-segmented mode once (every edge resident,
+// V <= kTlMaxSeg segments, tile words within free device memory); tl_state =
+// -1 when they cannot be built.
+template <class OffT>
+static int pr_tiled_build(asc_graph* g, PrState* p) {
+    if (p->tl_state != 0) return ASC_OK;
+    p->tl_state = -1;
+    const uint64_t V = g->h.V, E = g->h.E;
+    const uint64_t ns = (V + kTlSrcMask) >> kTlSrcBits, nd = (V + kTlDst - 1) >> kTlDstBits;
+    if (g->static_vertices != V || E == 0 || ns > kTlMaxSeg) return ASC_OK;
+    size_t free_b = 0, total_b = 0;
+    ASC_HIP(hipMemGetInfo(&free_b, &total_b));
+    const uint64_t need = 4 * E + 12 * nd * ns + 8 + (V + 31) / 32 * 4;
+    if (need + (uint64_t(1) << 30) > free_b) return ASC_OK;
+    hipStream_t s = g->ctx->s_main;
+    const OffT* off = (const OffT*)g->d_off;
+    const uint32_t* src = (const uint32_t*)g->d_edges;
+    const uint32_t thr = (uint32_t)p->heavy_thr;
+    DevTmp<uint32_t> d_cnt;
+    ASC_HIP(hipMalloc(&d_cnt.p, 4 * nd * ns));
+    hipLaunchKernelGGL((k_tl_count<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, (uint32_t)ns,
+                       d_cnt.p);
+    ASC_HIP(hipGetLastError());
+    std::vector<uint32_t> cnt(nd * ns);
+    ASC_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
+    ASC_HIP(hipStreamSynchronize(s));
+    std::vector<unsigned long long> toff(nd * ns + 1);
+    unsigned long long acc = 0;
+    for (size_t i = 0; i < cnt.size(); i++) {
+        toff[i] = acc;
+        acc += cnt[i];
+    }
+    toff[cnt.size()] = acc;
+    ASC_HIP(hipMalloc(&p->tl_off, 8 * toff.size()));
+    ASC_HIP(hipMemcpy(p->tl_off, toff.data(), 8 * toff.size(), hipMemcpyHostToDevice));
+    ASC_HIP(hipMalloc(&p->tl_edges, 4 * std::max<unsigned long long>(acc, 1)));
+    ASC_HIP(hipMalloc(&p->tl_heavy, 4 * ((V + 31) / 32)));
+    ASC_HIP(hipMemsetAsync(p->tl_heavy, 0, 4 * ((V + 31) / 32), s));
+    hipLaunchKernelGGL((k_tl_fill<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, (uint32_t)ns,
+                       (const unsigned long long*)p->tl_off, p->tl_edges, p->tl_heavy);
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipStreamSynchronize(s));
+    p->tl_nd = (uint32_t)nd;
+    p->tl_ns = (uint32_t)ns;
+    p->tl_edges_n = acc;
+    p->tl_state = 1;
+    return ASC_OK;
+}
+
 template <class OffT>
 static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, double* residual_out, asc_stats* st) {
     asc_ctx* ctx = g->ctx;
@@ -631,6 +815,12 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
     const bool blocked = o->method == ASC_PR_BLOCKED && p->pb_state == 1;
     if (o->method == ASC_PR_BLOCKED && !blocked) {
         set_last_error("propagation-blocked PageRank needs every edge resident, E < 2^32 and room for its slots");
+        return ASC_ERR_UNSUPPORTED;
+    }
+    if (o->method == ASC_PR_TILED) ASC_TRY(pr_tiled_build<OffT>(g, p));
+    const bool tiled = o->method == ASC_PR_TILED && p->tl_state == 1;
+    if (o->method == ASC_PR_TILED && !tiled) {
+        set_last_error("source-segmented PageRank needs every edge resident, V <= 2^31 and room for its tiles");
         return ASC_ERR_UNSUPPORTED;
     }
     ScopedEvent e0, e1;
@@ -659,7 +849,12 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         a.v_begin = 0;
         a.v_end = g->static_vertices;
         a.skip_heavy = 1;
-        if (blocked) {
+        if (tiled) {
+            hipLaunchKernelGGL(k_pr_tiled, dim3(p->tl_nd), dim3(kTlBlock), 0, s, a, (const uint32_t*)p->tl_edges,
+                               (const unsigned long long*)p->tl_off, p->tl_ns,
+                               p->n_heavy ? (const uint32_t*)p->tl_heavy : nullptr);
+            ASC_HIP(hipGetLastError());
+        } else if (blocked) {
             hipLaunchKernelGGL(k_pb_scatter, dim3(grid), dim3(kBlock), 0, s, p->pb_src, p->pb_pos, p->pb_edges,
                                a.contrib_in, p->pb_vals);
             hipLaunchKernelGGL((k_pb_gather<OffT>), dim3(p->pb_nbins), dim3(kPbBlock), 0, s, a, p->pb_bin_start,
@@ -782,10 +977,20 @@ int pagerank(asc_graph* g, const asc_pr_opts* o, float* rank_f32, double* rank_f
         set_last_error("PageRank needs a BCSC graph (in-CSR + out-degree)");
         return ASC_ERR_UNSUPPORTED;
     }
+    asc_stats local;
+    if (!st && g->ctx->cfg.verbose) st = &local;
+    int r;
     if (o->mode == ASC_PR_REFERENCE)
-        return g->off_bytes == 4 ? pr_reference<uint32_t>(g, o, rank_f64, st) : pr_reference<uint64_t>(g, o, rank_f64, st);
-    return g->off_bytes == 4 ? pr_normalised<uint32_t>(g, o, rank_f32, residual_out, st)
-                             : pr_normalised<uint64_t>(g, o, rank_f32, residual_out, st);
+        r = g->off_bytes == 4 ? pr_reference<uint32_t>(g, o, rank_f64, st) : pr_reference<uint64_t>(g, o, rank_f64, st);
+    else
+        r = g->off_bytes == 4 ? pr_normalised<uint32_t>(g, o, rank_f32, residual_out, st)
+                              : pr_normalised<uint64_t>(g, o, rank_f32, residual_out, st);
+    // per-phase timings in the reference's wording (CalculateOpt.cu:700-720)
+    if (r == ASC_OK && g->ctx->cfg.verbose && st)
+        printf("totalProcess time is %.3f ms\nstaticProcess time is %.3f ms\noverloadProcess time is %.3f ms\n"
+               "iterations : %u\n",
+               st->total_ms, st->static_ms, st->overload_ms, st->iterations);
+    return r;
 }
 
 }  // namespace asc

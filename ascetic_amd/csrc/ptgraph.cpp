@@ -18,7 +18,7 @@
 //   --overlap 0|1           resident steps between streamed chunks (SSSP, CC)
 //   --bfs td|auto           top-down only, or direction-optimizing when possible
 //   --cc lp|auto            label propagation only, or union-find when possible
-//   --pr-method auto|pull|blocked
+//   --pr-method auto|pull|blocked|tiled
 //   --type gen              write a synthetic graph: --gen rmat|er --scale s
 //                           --ef f --vertices n --samples m --seed x
 //                           --weighted 0|1 --wmax w --csc 0|1 --out file
@@ -62,7 +62,7 @@ void usage() {
     fprintf(stderr,
             "usage: ptgraph --input <graph file> [--type bfs|cc|sssp|pr|info] [--source N] [--adviseK f]\n"
             "               [--budget bytes] [--part-mb n] [--stream gather|zerocopy|hybrid] [--format fmt]\n"
-            "               [--values ref|zero] [--pr-mode ref|norm] [--pr-method auto|pull|blocked] [--iters n]\n"
+            "               [--values ref|zero] [--pr-mode ref|norm] [--pr-method auto|pull|blocked|tiled] [--iters n]\n"
             "               [--out file] [--verbose 0|1|2] [--dense f] [--overlap 0|1] [--bfs td|auto] [--cc lp|auto]\n"
             "               [--pack 0|1|2]  (streamed records: 8-byte / packed 4 or 5 bytes / always 5 bytes)\n"
             "       ptgraph --type gen --gen rmat|er [--scale s --ef f | --vertices n --samples m] [--seed x]\n"
@@ -193,6 +193,9 @@ int main(int argc, char** argv) {
         usage();
         return 2;
     }
+    // the algorithm implies the layout where only one serves it: PageRank
+    // reads the in-CSR file (GraphMeta.cuh:159-162)
+    if (fmt == ASC_FMT_AUTO && type == "pr") fmt = ASC_FMT_BCSC;
 
     asc_config cfg;
     asc_config_init(&cfg);
@@ -257,7 +260,7 @@ int main(int argc, char** argv) {
         const bool norm = a.get("--pr-mode", "ref") == "norm";
         o.mode = norm ? ASC_PR_NORMALISED : ASC_PR_REFERENCE;
         const std::string pm = a.get("--pr-method", "auto");
-        o.method = pm == "pull" ? ASC_PR_PULL : pm == "blocked" ? ASC_PR_BLOCKED : ASC_PR_AUTO;
+        o.method = pm == "pull" ? ASC_PR_PULL : pm == "blocked" ? ASC_PR_BLOCKED : pm == "tiled" ? ASC_PR_TILED : ASC_PR_AUTO;
         o.iterations = (int)a.num("--iters", 20);
         if (norm) {
             std::vector<float> rank(gi.num_vertices);

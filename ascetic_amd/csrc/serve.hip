@@ -760,6 +760,7 @@ struct SQuery {
     double t_sub = 0, t_adm = 0, t_done = 0;
     uint32_t lane = 0, steps = 0;
     bool done = false;
+    bool claimed = false;           // a thread waits on this ticket (asc_wait)
 };
 
 struct Engine {
@@ -1201,6 +1202,7 @@ extern "C" {
 int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** out) {
     if (!g || !out) return ASC_ERR_INVALID;
     *out = nullptr;
+    ASC_TRY(graph_ready(g));
     if (g->h.kind == ASC_KIND_CSC) return ASC_ERR_UNSUPPORTED;
     if (g->static_vertices < g->h.V) {
         set_last_error("serving needs a fully resident graph");
@@ -1242,6 +1244,7 @@ int asc_server_create(asc_graph* g, const asc_server_opts* opts, asc_server** ou
         return r;
     }
     const bool w = weighted, o4 = g->off_bytes == 4;
+    g->servers++;
     if (w) s->th = std::thread(o4 ? server_loop<uint32_t, WEdge> : server_loop<uint64_t, WEdge>, s);
     else s->th = std::thread(o4 ? server_loop<uint32_t, uint32_t> : server_loop<uint64_t, uint32_t>, s);
     *out = s;
@@ -1261,6 +1264,7 @@ int asc_server_destroy(asc_server* s) {
     // the ticket map owns every query not yet waited for (pending ones too;
     // the loop drains the queues before it ends)
     for (auto& kv : s->tickets) delete kv.second;
+    s->g->servers--;
     delete s;
     return ASC_OK;
 }
@@ -1304,6 +1308,13 @@ int asc_wait(asc_server* s, uint64_t ticket, asc_qstats* qs) {
     auto it = s->tickets.find(ticket);
     if (it == s->tickets.end()) return ASC_ERR_INVALID;
     SQuery* q = it->second;
+    // one waiter per ticket: a second thread waiting on the same ticket would
+    // find the query freed by the first
+    if (q->claimed) {
+        set_last_error("ticket %llu is already being waited for", (unsigned long long)ticket);
+        return ASC_ERR_INVALID;
+    }
+    q->claimed = true;
     s->cv_done.wait(lk, [&] { return q->done; });
     if (qs) {
         qs->submit_to_admit_ms = q->t_adm - q->t_sub;
@@ -1312,7 +1323,7 @@ int asc_wait(asc_server* s, uint64_t ticket, asc_qstats* qs) {
         qs->lane = q->lane;
     }
     const int err = q->t_done > 0 ? ASC_OK : s->error;
-    s->tickets.erase(it);
+    s->tickets.erase(ticket);   // `it` may be stale: the map can rehash while this thread waits
     delete q;
     return err;
 }

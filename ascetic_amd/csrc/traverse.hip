@@ -1243,6 +1243,11 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
         if (st) {
             st->reached = reached;
             st->reached_degree_sum = dsum;
+            // BFS is level-synchronous: each reached vertex is active in
+            // exactly one iteration, so the reference's nodeSum (actives summed
+            // over iterations, CalculateOpt.cu:183-190) is the reached count,
+            // whichever direction each level took
+            if (algo == ALG_BFS) st->node_sum = reached;
         }
     }
     if (ctx->cfg.verbose && st) {

@@ -56,7 +56,9 @@ int asc_abi_version(void);
  *   LEGACY32 u32 V, u32 E, u32 off[V], u32 dst[E]          (bfs.cu:294-304)
  *   LEGACY32W u32 V, u32 E, u32 off[V], {u32 dst, u32 w}[E] (sssp.cu:176-186)
  * Offsets carry no off[V] sentinel; the last degree is E - off[V-1]
- * (GraphMeta.cuh:226).  ASC_FMT_AUTO decides by the file size. */
+ * (GraphMeta.cuh:226).  ASC_FMT_AUTO tries, in the enum's order, every layout
+ * whose size matches the header and keeps the first that validates (BWCSR
+ * and BCSC have equal sizes when V == E). */
 typedef enum {
     ASC_FMT_AUTO = 0,
     ASC_FMT_BCSR = 1,
@@ -123,7 +125,9 @@ typedef struct {
                                     on the device, two pinned staging buffers
                                     on the host); 0 = 64 MiB                  */
     uint64_t device_mem_limit;   /* > 0: pretend the device has this much free
-                                    memory when sizing (testing); 0 = query   */
+                                    memory when sizing, and refuse (ASC_ERR_
+                                    NOMEM) a graph whose planned device copies
+                                    exceed it (testing); 0 = query           */
     int gather_threads;          /* host gather threads; 0 = min(16, cores)   */
     int stream_mode;             /* asc_stream_mode                           */
     int value_convention;        /* asc_value_convention                      */
@@ -224,7 +228,14 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info);
 int asc_write_graph(const char* path, int format, const asc_csr_view* view);
 
 /* Re-plans the static / on-demand split of a loaded graph with a new byte
- * budget (0 = automatic rule) or advise rate; uploads the new static prefix. */
+ * budget (0 = automatic rule) or advise rate; uploads the new static prefix.
+ * The budget and rate apply to this graph only (graphs created later in the
+ * context keep the context's).  A plan whose device copies cannot fit is
+ * refused before anything changes (the graph keeps its current split); if a
+ * HIP allocation or copy fails after the old copies were freed, every later
+ * call on the graph returns ASC_ERR_STATE until a re-plan succeeds.  Refused
+ * with ASC_ERR_STATE while a server runs over the graph (so is
+ * asc_graph_destroy). */
 int asc_graph_replan(asc_graph* g, uint64_t edge_budget_bytes, double advise_rate);
 
 /* ---- per-run statistics (phase names follow CalculateOpt.cu:30-36) ------ */
@@ -293,9 +304,15 @@ typedef struct {
 typedef enum {
     ASC_PR_AUTO = 0,     /* the pull (the faster one measured on MI355X)     */
     ASC_PR_PULL = 1,     /* G-lane pull over the in-CSR                      */
-    ASC_PR_BLOCKED = 2   /* propagation-blocked (scatter to slots by
+    ASC_PR_BLOCKED = 2,  /* propagation-blocked (scatter to slots by
                             destination bin, LDS sums per bin); needs every
                             edge resident, else ASC_ERR_UNSUPPORTED         */
+    ASC_PR_TILED = 3     /* source-segmented: in-edges regrouped once into
+                            (destination block, source segment) tiles, a
+                            workgroup sums one block in LDS while walking
+                            the segments, whose contribution slices stay in
+                            L2; needs every edge resident and V <= 2^31,
+                            else ASC_ERR_UNSUPPORTED                        */
 } asc_pr_method;
 void asc_pr_opts_init(asc_pr_opts* o);
 
@@ -453,7 +470,9 @@ int asc_server_destroy(asc_server* s);
  * unreached V+1) before the query completes.  *ticket identifies the query.
  * Returns ASC_ERR_BUSY when queue_capacity queries are outstanding. */
 int asc_submit(asc_server* s, int kind, uint32_t source, uint32_t* out, uint64_t* ticket);
-/* Thread-safe.  Blocks until the query completed and releases the ticket. */
+/* Thread-safe.  Blocks until the query completed and releases the ticket.
+ * One thread waits per ticket: a second asc_wait on a ticket that another
+ * thread is waiting for returns ASC_ERR_INVALID, as does a released ticket. */
 int asc_wait(asc_server* s, uint64_t ticket, asc_qstats* qs);
 /* Blocks until every submitted query completed. */
 int asc_server_drain(asc_server* s);

@@ -25,6 +25,12 @@
 static int g_fault = 0;
 void orc_set_fault(int fault) { g_fault = fault; }
 
+/* Bounded CPU-baseline samples: the restated traversal stops after this many
+ * iterations (0 = run to the end); its values are then partial. */
+static int g_iter_limit = 0;
+void orc_set_iteration_limit(int n) { g_iter_limit = n > 0 ? n : 0; }
+void orc_set_num_threads(int n) { if (n > 0) omp_set_num_threads(n); }
+
 enum { ALG_BFS = 0, ALG_SSSP = 1, ALG_CC = 2 };
 
 static inline uint32_t edst(const orc_graph* g, uint64_t i) { return g->dst[i * (uint64_t)g->stride]; }
@@ -135,7 +141,7 @@ static int traverse_ref(const orc_graph* g, int algo, uint32_t src, uint64_t S, 
     for (uint64_t v = 0; v < V; v++) count += active[v];
     local.node_sum = count;
 
-    while (count) {
+    while (count && !(g_iter_limit > 0 && local.iterations >= (uint32_t)g_iter_limit)) {
         local.iterations++;
         /* setStaticAndOverloadLabelBool + scan + compaction (CalculateOpt.cu:63-91):
          * lists in increasing vertex order. */
@@ -336,6 +342,38 @@ int orc_pr_norm(const orc_graph* g, const uint32_t* outdeg, double damping, int 
     free(nr);
     free(contrib);
     return 0;
+}
+
+/* fillEdgeArrByMultiThread (GraphMeta.cuh:469-514): the adjacency of each
+ * listed vertex copied, in list order, into one contiguous buffer; the list
+ * is cut into `nthreads` chunks of ceil(n / nthreads) vertices (one thread
+ * when n < 50), each copied by its own thread.  Records are copied whole
+ * (stride * 4 bytes: {dst, w} when the graph is AoS).  Returns the edges
+ * copied; *copy_ms is the wall time of the threaded copy alone (the
+ * reference's fillProcess), the degree prefix excluded. */
+long long orc_gather_ref(const orc_graph* g, const uint32_t* list, uint64_t n, int nthreads, uint32_t* out,
+                         double* copy_ms) {
+    uint64_t* pre = (uint64_t*)malloc(sizeof(uint64_t) * (n + 1));
+    if (!pre) return -2;
+    pre[0] = 0;
+    for (uint64_t i = 0; i < n; i++) pre[i + 1] = pre[i] + degree(g, list[i]);
+    const int nt = n < 50 ? 1 : (nthreads > 0 ? nthreads : 20);
+    const uint64_t chunk = (n + (uint64_t)nt - 1) / (uint64_t)nt;
+    const int words = g->stride;
+    const double t0 = omp_get_wtime();
+#pragma omp parallel num_threads(nt)
+    {
+        const uint64_t t = (uint64_t)omp_get_thread_num();
+        const uint64_t lo = t * chunk < n ? t * chunk : n, hi = lo + chunk < n ? lo + chunk : n;
+        for (uint64_t i = lo; i < hi; i++) {
+            const uint64_t from = g->off[list[i]], d = pre[i + 1] - pre[i];
+            memcpy(out + pre[i] * (uint64_t)words, g->dst + from * (uint64_t)words, 4 * d * (uint64_t)words);
+        }
+    }
+    if (copy_ms) *copy_ms = (omp_get_wtime() - t0) * 1e3;
+    const long long total = (long long)pre[n];
+    free(pre);
+    return total;
 }
 
 /* ---- textbook algorithms ----------------------------------------------- */

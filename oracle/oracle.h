@@ -106,6 +106,18 @@ int orc_split_reference(const uint64_t* prefix, const uint32_t* deg, uint32_t n,
                         uint64_t total, uint64_t part_size, uint32_t* starts,
                         uint32_t* counts, uint64_t* edges, int cap);
 
+/* CPU-baseline helpers.  orc_set_iteration_limit(n): the restated
+ * traversals (orc_*_ref) stop after n iterations (0 = no limit), for bounded
+ * timing samples; orc_set_num_threads(n): OpenMP threads of the functions
+ * without an nthreads argument (orc_pr_norm).  orc_gather_ref restates
+ * fillEdgeArrByMultiThread (GraphMeta.cuh:469-514): the adjacency records of
+ * list[0..n) copied into `out` by nthreads threads (20 in the reference; one
+ * when n < 50); returns the edges copied and the copy's wall time. */
+void orc_set_iteration_limit(int n);
+void orc_set_num_threads(int n);
+long long orc_gather_ref(const orc_graph* g, const uint32_t* list, uint64_t n, int nthreads, uint32_t* out,
+                         double* copy_ms);
+
 /* Fault injection for the oracle's own tests: 0 = none; any other value
  * plants one deliberate mistake (listed in oracle.c) so a test can show the
  * checks catch it. */

@@ -66,11 +66,15 @@ def load():
                                    C.POINTER(Plan), C.POINTER(C.c_uint8)],
             "orc_split_reference": [u64p, u32p, C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p, u64p, C.c_int],
             "orc_set_fault": [C.c_int],
+            "orc_set_iteration_limit": [C.c_int],
+            "orc_set_num_threads": [C.c_int],
+            "orc_gather_ref": [G, u32p, C.c_uint64, C.c_int, u32p, f64p],
         }
+        void = ("orc_set_fault", "orc_set_iteration_limit", "orc_set_num_threads")
         for name, args in sig.items():
             f = getattr(lib, name)
             f.argtypes = args
-            f.restype = None if name == "orc_set_fault" else C.c_int
+            f.restype = None if name in void else (C.c_longlong if name == "orc_gather_ref" else C.c_int)
         _lib = lib
     return _lib
 
@@ -190,6 +194,50 @@ def split_reference(prefix, deg, part_size, cap=1 << 16):
     if np_ < 0:
         return None
     return s[:np_], c[:np_], e[:np_]
+
+
+def gather_ref(csr, vertices, nthreads=20, aos=None):
+    """fillEdgeArrByMultiThread restated: copies the adjacency records of
+    `vertices` into one buffer with `nthreads` threads.  aos: an (E, 2) u32
+    array of {dst, w} records to copy instead of csr.dst (8-byte records).
+    Returns (buffer, copy milliseconds)."""
+    lst = np.ascontiguousarray(vertices, dtype=np.uint32)
+    if aos is not None:
+        aos = np.ascontiguousarray(aos, dtype=np.uint32)
+        g = Graph()
+        g.V, g.E, g.off = csr.V, csr.E, csr.g.off
+        g.dst = aos.ctypes.data_as(u32p)
+        g.w = None
+        g.stride = 2
+        words = 2
+    else:
+        g, words = csr.g, 1
+    deg = (csr.off[lst.astype(np.int64) + 1] - csr.off[lst.astype(np.int64)]).sum() if len(lst) else 0
+    out = np.empty(max(int(deg) * words, 1), dtype=np.uint32)
+    ms = C.c_double()
+    n = load().orc_gather_ref(C.byref(g), lst.ctypes.data_as(u32p), len(lst), nthreads, out.ctypes.data_as(u32p),
+                              C.byref(ms))
+    if n < 0:
+        raise RuntimeError(f"orc_gather_ref failed ({n})")
+    return out[:int(deg) * words], ms.value
+
+
+class iteration_limit:
+    """Context manager bounding the restated traversals to n iterations
+    (timing samples; values are partial)."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        load().orc_set_iteration_limit(self.n)
+
+    def __exit__(self, *exc):
+        load().orc_set_iteration_limit(0)
+
+
+def set_num_threads(n):
+    load().orc_set_num_threads(n)
 
 
 class fault:

@@ -54,6 +54,30 @@ def test_oracle_matches_known_answer(case):
         assert got.tolist() == case["expected"]
 
 
+def test_huge_weight_value_is_the_saturating_one():
+    """Which oracle function defines SSSP when a value plus a weight leaves
+    32 bits.  The reference adds in plain 32-bit `unsigned` arithmetic
+    (gpu_kernels.cuh:496-499), and so does the oracle's restatement of it,
+    orc_sssp_ref: 1 + (2^32 - 1) wraps to 0, which is accepted as smaller than
+    V + 1, and the restatement answers distances below the true ones (an
+    overflow of the reference, not a graph property).  The library adds with
+    saturation instead, so its value is the exact distance capped at V + 1:
+    orc_dijkstra's (64-bit sums, min(1 + d, V + 1)).  The golden case
+    huge_weight3_sssp_from_0 records that value, and the GPU tests above hold
+    the library to it; below 2^32 - V - 1 the two additions agree."""
+    case = next(c for c in CASES if c["name"] == "huge_weight3_sssp_from_0")
+    off, dst, w = build(case)
+    csr = O.CSR(off, dst, w)
+    assert O.dijkstra(csr, 0).tolist() == case["expected"] == [1, 4, 4]
+    wrapped, _ = O.sssp_ref(csr, 0)
+    assert wrapped.tolist() == [1, 0, 1]          # 1 + (2^32 - 1) = 0 (mod 2^32), then 0 + 1
+    # one below the wrap point the restatement and Dijkstra agree
+    e = [[0, 1, 2 ** 32 - 1 - 4], [1, 2, 1]]
+    off, dst, w = G.csr_from_edges(3, [(a, b) for a, b, _ in e], [c for _, _, c in e])
+    csr = O.CSR(off, dst, w)
+    assert O.sssp_ref(csr, 0)[0].tolist() == O.dijkstra(csr, 0).tolist() == [1, 4, 4]
+
+
 @pytest.mark.gpu
 @pytest.mark.parametrize("on_demand", [False, True], ids=["resident", "on_demand"])
 @pytest.mark.parametrize("case", CASES, ids=[c["name"] for c in CASES])

@@ -138,6 +138,24 @@ def test_bcsc_roundtrip(tmp_path):
     assert np.array_equal(h.out_degree(), od) and np.array_equal(h.dst(), isrc)
 
 
+def test_auto_format_bcsc_with_v_equal_e(tmp_path):
+    """A .bcsc with V == E has the size of a .bwcsr with the same header; the
+    automatic layout choice keeps trying size-matching layouts until one
+    validates, so the file still reads as the in-CSR it is."""
+    V = 64
+    off = np.arange(V + 1, dtype=np.uint64)                 # a directed cycle: one out-edge each
+    dst = ((np.arange(V) + 1) % V).astype(np.uint32)
+    ioff, isrc = G.transpose(off, dst, V)
+    od = np.ones(V, np.uint32)
+    p = str(tmp_path / "cycle.bcsc")
+    A.write_graph(p, _abi.FMT_BCSC, ioff[:-1], isrc, _abi.KIND_CSC, out_degree=od)
+    assert os.path.getsize(p) == 16 + 8 * V + 8 * V        # = the BWCSR size for V == E
+    h = A.read_graph(p)
+    assert h.kind == _abi.KIND_CSC
+    assert np.array_equal(h.offsets(), ioff) and np.array_equal(h.dst(), isrc)
+    assert np.array_equal(h.out_degree(), od)
+
+
 def test_reader_rejects_bad_files(tmp_path):
     off, dst, _ = G.path_graph(10)
     p = str(tmp_path / "g.bcsr")

@@ -319,3 +319,32 @@ def test_planted_faults_are_caught():
     assert _fault_caught(4, pr_ref_check)
     assert _fault_caught(5, pr_norm_check)
     assert _fault_caught(6, part_check)
+
+
+def test_gather_ref_copies_adjacencies_in_list_order():
+    """fillEdgeArrByMultiThread restated: the adjacency records of the listed
+    vertices, concatenated in list order, whatever the thread count."""
+    off, dst, w = G.random_graph(500, 3000, 41, symmetric=True, weighted=True)
+    csr = O.CSR(off, dst, w)
+    lst = np.array([7, 3, 499, 0, 250, 3], np.uint32)
+    expect = np.concatenate([dst[off[v]:off[v + 1]] for v in lst])
+    for nt in (1, 3, 20):
+        got, ms = O.gather_ref(csr, lst, nt)
+        assert np.array_equal(got, expect) and ms >= 0
+    aos = G.wedges(dst, w)
+    big = np.arange(0, 500, 2, dtype=np.uint32)          # >= 50 vertices: 20 threads
+    got, _ = O.gather_ref(csr, big, 20, aos=aos)
+    assert np.array_equal(got.reshape(-1, 2), np.concatenate([aos[off[v]:off[v + 1]] for v in big]))
+
+
+def test_iteration_limit_bounds_the_restatement():
+    """With a limit of k iterations, BFS on a path reaches exactly the first
+    k + 1 vertices; without it, the whole path."""
+    off, dst, _ = G.path_graph(20)
+    csr = O.CSR(off, dst)
+    with O.iteration_limit(3):
+        lv, st = O.bfs_ref(csr, 0)
+    assert st["iterations"] == 3
+    assert lv[:4].tolist() == [1, 2, 3, 4] and (lv[4:] == 21).all()
+    lv, st = O.bfs_ref(csr, 0)
+    assert lv.tolist() == list(range(1, 21))

@@ -51,8 +51,8 @@ def test_pr_normalised_hub_and_dangling(gpu_ctx):
     assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
 
 
-@pytest.mark.parametrize("case", ["sym", "dir", "hub", "rmat", "multi_bin"])
-@pytest.mark.parametrize("method", [_abi.PR_PULL, _abi.PR_BLOCKED])
+@pytest.mark.parametrize("case", ["sym", "dir", "hub", "rmat", "multi_bin", "multi_segment"])
+@pytest.mark.parametrize("method", [_abi.PR_PULL, _abi.PR_BLOCKED, _abi.PR_TILED])
 def test_pr_blocked_and_pull_agree(gpu_ctx, case, method):
     """Propagation-blocked and pulled normalised PageRank both meet the L1
     bound: symmetric graphs (out-CSR = in-CSR), directed ones (transposed on
@@ -69,9 +69,12 @@ def test_pr_blocked_and_pull_agree(gpu_ctx, case, method):
         h = A.rmat(13, 16, seed=8)
         off, dst = h.offsets().copy(), h.dst().copy()
         V = len(off) - 1
-    else:
+    elif case == "multi_bin":
         V = 100000
         off, dst, _ = G.random_graph(V, 400000, 32, symmetric=False)
+    else:   # three source segments of the tiled schedule, 37 destination blocks
+        V = 600000
+        off, dst, _ = G.random_graph(V, 1500000, 34, symmetric=False)
     ioff, isrc, od = csc_of(off, dst, V)
     ref, ref_res = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 20)
     with gpu_ctx.from_arrays(ioff, isrc, out_degree=od) as g:
@@ -82,15 +85,35 @@ def test_pr_blocked_and_pull_agree(gpu_ctx, case, method):
     assert np.allclose(res, ref_res, rtol=1e-3, atol=1e-7)
 
 
+def test_pr_er_2p22_normalised(gpu_ctx):
+    """Erdos-Renyi with V = 2^22 and 2^24 undirected pairs (2^25 directed
+    edges): the size where FP32 ranks and sums start to lose digits; every
+    schedule stays within the L1 bound of the FP64 oracle."""
+    h = A.erdos_renyi(1 << 22, 1 << 24, seed=5)
+    off, dst = h.offsets().copy(), h.dst().copy()
+    V = len(off) - 1
+    od = np.diff(off).astype(np.uint32)
+    # symmetric: the out-CSR is also the in-CSR
+    ref, _ = O.pr_norm(O.CSR(off, dst), od, 0.85, 20)
+    with gpu_ctx.from_arrays(off, dst, out_degree=od) as g:
+        for method in (_abi.PR_PULL, _abi.PR_TILED, _abi.PR_AUTO):
+            r, _, st = g.pagerank(iterations=20, method=method)
+            l1 = float(np.abs(r.astype(np.float64) - ref).sum())
+            print("method", method, "L1", l1, flush=True)
+            assert l1 <= TOL_L1, method
+    h.free()
+
+
 def test_pr_blocked_needs_resident_graph():
     off, dst, _ = G.random_graph(3000, 12000, 33, symmetric=True)
     ioff, isrc, od = csc_of(off, dst, 3000)
     E = int(ioff[-1])
     with A.Context(edge_budget_bytes=4 * (E // 3), part_bytes=4 * 4096) as ctx, \
             ctx.from_arrays(ioff, isrc, out_degree=od) as g:
-        with pytest.raises(A.AscError) as e:
-            g.pagerank(iterations=5, method=_abi.PR_BLOCKED)
-        assert e.value.code == _abi.ERR_UNSUPPORTED
+        for method in (_abi.PR_BLOCKED, _abi.PR_TILED):
+            with pytest.raises(A.AscError) as e:
+                g.pagerank(iterations=5, method=method)
+            assert e.value.code == _abi.ERR_UNSUPPORTED
         ref, _ = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 5)
         r, _, _ = g.pagerank(iterations=5)   # AUTO: the streamed pull
         assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1
