@@ -351,13 +351,14 @@ class Graph:
         return self._run(_abi.load().asc_cc, output=output)
 
     def pagerank(self, mode=_abi.PR_NORMALISED, damping=0.85, iterations=20, threshold=0.001,
-                 max_iterations=0, method=_abi.PR_AUTO):
+                 max_iterations=0, method=_abi.PR_AUTO, tile_src_bits=0, tile_dst_bits=0):
         lib = _abi.load()
         o = _abi.PrOpts()
         lib.asc_pr_opts_init(C.byref(o))
         o.mode, o.damping, o.iterations, o.threshold, o.max_iterations = (
             mode, damping, iterations, threshold, max_iterations)
         o.method = method
+        o.tile_src_bits, o.tile_dst_bits = tile_src_bits, tile_dst_bits
         V = self.num_vertices
         st = _abi.Stats()
         if mode == _abi.PR_NORMALISED:

@@ -130,6 +130,7 @@ class Stats(C.Structure):
         ("host_cores", C.c_int),
         ("path", C.c_int),
         ("kern", KernelStats * 6),
+        ("reused_edges", C.c_uint64),
     ]
 
     def as_dict(self):
@@ -148,6 +149,8 @@ class PrOpts(C.Structure):
         ("threshold", C.c_double),
         ("max_iterations", C.c_int),
         ("method", C.c_int),
+        ("tile_src_bits", C.c_int),
+        ("tile_dst_bits", C.c_int),
     ]
 
 

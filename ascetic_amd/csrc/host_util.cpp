@@ -144,6 +144,8 @@ void asc_pr_opts_init(asc_pr_opts* o) {
     o->threshold = 0.001;
     o->max_iterations = 0;
     o->method = ASC_PR_AUTO;
+    o->tile_src_bits = 0;
+    o->tile_dst_bits = 0;
 }
 
 void asc_gen_params_init(asc_gen_params* p) {

@@ -67,7 +67,7 @@ struct PrState {
     // tile (d, s) at tl_edges[tl_off[d * tl_ns + s] ...], one u32 word per
     // edge (block-local destination << kTlSrcBits | segment-local source)
     int tl_state = 0;             // 0 not built, 1 built, -1 not possible
-    uint32_t tl_nd = 0, tl_ns = 0;
+    uint32_t tl_sb = 0, tl_db = 0, tl_nd = 0, tl_ns = 0;
     uint64_t tl_edges_n = 0;
     uint32_t* tl_edges = nullptr;
     unsigned long long* tl_off = nullptr;
@@ -424,107 +424,119 @@ __global__ __launch_bounds__(kPbBlock) void k_pb_gather(PrArgs a, const unsigned
 // The pull gathers one random 4-byte contribution per in-edge from an array
 // of 4V bytes (256 MB at V = 2^26, beyond the 4 MB L2 of an XCD and as large
 // as the MALL), so almost every gather moves a whole line from HBM.  Here the
-// light in-edges are regrouped once into tiles by (destination block of
-// 2^14 vertices, source segment of 2^18 vertices).  One workgroup owns a
-// destination block: it keeps the block's sums in LDS (FP32, 64 KB) and walks
-// the segments in order, so the gathers of a tile read one 1 MB slice of the
-// contributions, and the workgroups resident on an XCD, all walking the
-// segments in the same order, keep that slice in the XCD's L2.  A tile edge
-// is one u32 word (block-local destination << 18 | segment-local source):
-// 4 streamed bytes per edge instead of a 4-byte id plus a line per gather.
-// The last pass over the block finishes its vertices (rank, contribution,
-// residual, dangling mass) as the pull does.  Destinations with in-degree >=
-// the heavy threshold keep the FP64 pull (k_pr_pull_heavy).
-constexpr int kTlDstBits = 14;
-constexpr int kTlSrcBits = 18;
-constexpr uint32_t kTlDst = 1u << kTlDstBits;              // destinations per block
-constexpr uint32_t kTlSrcMask = (1u << kTlSrcBits) - 1;    // sources per segment - 1
-constexpr uint32_t kTlMaxSeg = 8192;                       // build histograms in LDS: V <= 2^31
+// light in-edges are regrouped once into tiles by (destination block of 2^db
+// vertices, source segment of 2^sb vertices).  One workgroup owns a
+// destination block at a time: it keeps the block's sums in LDS (FP32) and
+// walks the segments in order, so the gathers of a tile read one 4 * 2^sb
+// byte slice of the contributions, and the workgroups resident on an XCD,
+// walking the segments in the same order, share that slice in the XCD's L2.
+// A tile edge is one u32 word (block-local destination << sb | segment-local
+// source).  The grid is persistent: workgroup b takes blocks b, b + G, ...,
+// so the resident workgroups start together and stay near each other's
+// segment.  The last pass over a block finishes its vertices (rank,
+// contribution, residual, dangling mass) as the pull does.  Destinations with
+// in-degree >= the heavy threshold keep the FP64 pull (k_pr_pull_heavy).
+constexpr int kTlMaxDstBits = 14;                          // LDS sums: 16384 FP32 (64 KB)
+constexpr uint32_t kTlMaxSeg = 8192;                       // build histograms in LDS
 constexpr int kTlBlock = 1024;                             // workgroup of the tiled pull
+
+struct TileShape {
+    uint32_t sb, db;            // source segment / destination block bits
+    uint32_t ns, nd;            // segments, blocks
+};
 
 // cnt[d * ns + s] = light in-edges of destination block d from segment s.
 template <class OffT>
 __global__ __launch_bounds__(kBlock) void k_tl_count(const OffT* off, const uint32_t* src, uint64_t V,
-                                                     uint32_t heavy_thr, uint32_t ns, uint32_t* cnt) {
+                                                     uint32_t heavy_thr, TileShape ts, uint32_t* cnt) {
     __shared__ uint32_t h[kTlMaxSeg];
     const uint64_t d = blockIdx.x;
-    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) h[i] = 0;
+    for (uint32_t i = threadIdx.x; i < ts.ns; i += kBlock) h[i] = 0;
     __syncthreads();
-    const uint64_t v0 = d << kTlDstBits, v1 = min<uint64_t>(V, v0 + kTlDst);
+    const uint64_t v0 = d << ts.db, v1 = min<uint64_t>(V, v0 + (1ull << ts.db));
     for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
         const OffT b = off[v], e = off[v + 1];
         if ((uint64_t)(e - b) >= heavy_thr) continue;
-        for (OffT k = b; k < e; k++) atomicAdd(&h[src[k] >> kTlSrcBits], 1u);
+        for (OffT k = b; k < e; k++) atomicAdd(&h[src[k] >> ts.sb], 1u);
     }
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) cnt[d * ns + i] = h[i];
+    for (uint32_t i = threadIdx.x; i < ts.ns; i += kBlock) cnt[d * ts.ns + i] = h[i];
 }
 
 // Writes the tile words of destination block d (positions within a tile in
 // arrival order) and marks its heavy destinations in `heavy_bits`.
 template <class OffT>
 __global__ __launch_bounds__(kBlock) void k_tl_fill(const OffT* off, const uint32_t* src, uint64_t V,
-                                                    uint32_t heavy_thr, uint32_t ns, const unsigned long long* toff,
-                                                    uint32_t* tiles, uint32_t* heavy_bits) {
+                                                    uint32_t heavy_thr, TileShape ts,
+                                                    const unsigned long long* toff, uint32_t* tiles,
+                                                    uint32_t* heavy_bits) {
     __shared__ uint32_t cur[kTlMaxSeg];
     const uint64_t d = blockIdx.x;
-    for (uint32_t i = threadIdx.x; i < ns; i += kBlock) cur[i] = 0;
+    for (uint32_t i = threadIdx.x; i < ts.ns; i += kBlock) cur[i] = 0;
     __syncthreads();
-    const uint64_t v0 = d << kTlDstBits, v1 = min<uint64_t>(V, v0 + kTlDst);
-    const unsigned long long* t = toff + d * ns;
+    const uint64_t v0 = d << ts.db, v1 = min<uint64_t>(V, v0 + (1ull << ts.db));
+    const unsigned long long* t = toff + d * ts.ns;
+    const uint32_t smask = (1u << ts.sb) - 1;
     for (uint64_t v = v0 + threadIdx.x; v < v1; v += kBlock) {
         const OffT b = off[v], e = off[v + 1];
         if ((uint64_t)(e - b) >= heavy_thr) {
             atomicOr(&heavy_bits[v >> 5], 1u << (v & 31));
             continue;
         }
-        const uint32_t dl = (uint32_t)(v - v0) << kTlSrcBits;
+        const uint32_t dl = (uint32_t)(v - v0) << ts.sb;
         for (OffT k = b; k < e; k++) {
             const uint32_t u = src[k];
-            const uint32_t s = u >> kTlSrcBits;
-            tiles[t[s] + atomicAdd(&cur[s], 1u)] = dl | (u & kTlSrcMask);
+            const uint32_t s = u >> ts.sb;
+            tiles[t[s] + atomicAdd(&cur[s], 1u)] = dl | (u & smask);
         }
     }
 }
 
-// One workgroup per destination block; grid = number of blocks.
+// Persistent: workgroup b takes destination blocks b, b + G, b + 2G, ...
+// (G = gridDim.x).
 __global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t* __restrict__ tiles,
-                                                       const unsigned long long* __restrict__ toff, uint32_t ns,
+                                                       const unsigned long long* __restrict__ toff, TileShape ts,
                                                        const uint32_t* heavy_bits) {
-    __shared__ float acc[kTlDst];
+    __shared__ float acc[1u << kTlMaxDstBits];
     __shared__ double s_r[kTlBlock / kWave], s_d[kTlBlock / kWave];
-    const uint64_t d = blockIdx.x;
-    for (uint32_t i = threadIdx.x; i < kTlDst; i += kTlBlock) acc[i] = 0.0f;
-    __syncthreads();
-    const unsigned long long* t = toff + d * ns;
-    unsigned long long k0 = t[0];
-    for (uint32_t s = 0; s < ns; s++) {
-        const unsigned long long k1 = t[s + 1];
-        const float* cseg = a.contrib_in + ((uint64_t)s << kTlSrcBits);
-        // four words per lane per pass: the words, then their gathers, then
-        // the LDS adds (lanes past the tile's end re-read its first word)
-        for (unsigned long long k = k0 + threadIdx.x; k < k1; k += 4 * kTlBlock) {
-            uint32_t w[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) w[j] = tiles[k + j * kTlBlock < k1 ? k + j * kTlBlock : k0];
-            float c[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) c[j] = cseg[w[j] & kTlSrcMask];
-#pragma unroll
-            for (int j = 0; j < 4; j++)
-                if (k + j * kTlBlock < k1) atomicAdd(&acc[w[j] >> kTlSrcBits], c[j]);
-        }
-        k0 = k1;
-    }
-    __syncthreads();
+    const uint32_t G = gridDim.x;
+    const uint32_t nb = blockIdx.x < ts.nd ? (ts.nd - blockIdx.x + G - 1) / G : 0;   // blocks of this workgroup
+    const uint32_t dsz = 1u << ts.db, smask = (1u << ts.sb) - 1;
     const double base = (1.0 - a.damping64) / (double)a.V + a.damping64 * (*a.dang_in) / (double)a.V;
     double res = 0, dang = 0;
-    const uint64_t v0 = d << kTlDstBits;
-    for (uint32_t i = threadIdx.x; i < kTlDst; i += kTlBlock) {
-        const uint64_t v = v0 + i;
-        if (v >= a.V) break;
-        if (heavy_bits && (heavy_bits[v >> 5] >> (v & 31) & 1u)) continue;
-        pr_finish_vertex(a, v, (double)acc[i], base, res, dang);
+    for (uint32_t r = 0; r < nb; r++) {
+        const uint64_t d = blockIdx.x + (uint64_t)r * G;
+        for (uint32_t i = threadIdx.x; i < dsz; i += kTlBlock) acc[i] = 0.0f;
+        __syncthreads();
+        const unsigned long long* t = toff + d * ts.ns;
+        unsigned long long k0 = t[0];
+        for (uint32_t s = 0; s < ts.ns; s++) {
+            const unsigned long long k1 = t[s + 1];
+            const float* cseg = a.contrib_in + ((uint64_t)s << ts.sb);
+            // four words per lane per pass: the words, then their gathers,
+            // then the LDS adds (lanes past the tile's end re-read its first word)
+            for (unsigned long long k = k0 + threadIdx.x; k < k1; k += 4 * kTlBlock) {
+                uint32_t w[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) w[j] = tiles[k + j * kTlBlock < k1 ? k + j * kTlBlock : k0];
+                float c[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) c[j] = cseg[w[j] & smask];
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    if (k + j * kTlBlock < k1) atomicAdd(&acc[w[j] >> ts.sb], c[j]);
+            }
+            k0 = k1;
+        }
+        __syncthreads();
+        const uint64_t v0 = d << ts.db;
+        for (uint32_t i = threadIdx.x; i < dsz; i += kTlBlock) {
+            const uint64_t v = v0 + i;
+            if (v >= a.V) break;
+            if (heavy_bits && (heavy_bits[v >> 5] >> (v & 31) & 1u)) continue;
+            pr_finish_vertex(a, v, (double)acc[i], base, res, dang);
+        }
+        __syncthreads();
     }
     res = wave_sum(res);
     dang = wave_sum(dang);
@@ -534,12 +546,12 @@ __global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t*
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double r = 0, dd = 0;
+        double rr = 0, dd = 0;
         for (int i = 0; i < kTlBlock / kWave; i++) {
-            r += s_r[i];
+            rr += s_r[i];
             dd += s_d[i];
         }
-        if (r != 0) atomicAdd(a.residual, r);
+        if (rr != 0) atomicAdd(a.residual, rr);
         if (dd != 0) atomicAdd(a.dang_out, dd);
     }
 }
@@ -746,16 +758,25 @@ static int pr_blocked_build(asc_graph* g, PrState* p) {
     return ASC_OK;
 }
 
-// Builds the tiles of the source-segmented mode once (every edge resident,
-// V <= kTlMaxSeg segments, tile words within free device memory); tl_state =
-// -1 when they cannot be built.
+// Builds the tiles of the source-segmented mode for segments of 2^sb and
+// blocks of 2^db vertices (every edge resident, at most kTlMaxSeg segments,
+// sb + db <= 32, tile words within free device memory); rebuilt when the
+// shape changes.  tl_state = -1 when they cannot be built.
 template <class OffT>
-static int pr_tiled_build(asc_graph* g, PrState* p) {
-    if (p->tl_state != 0) return ASC_OK;
+static int pr_tiled_build(asc_graph* g, PrState* p, uint32_t sb, uint32_t db) {
+    if (p->tl_state == 1 && p->tl_sb == sb && p->tl_db == db) return ASC_OK;
+    auto f = [](void* x) {
+        if (x) (void)hipFree(x);
+    };
+    f(p->tl_edges); f(p->tl_off); f(p->tl_heavy);
+    p->tl_edges = nullptr;
+    p->tl_off = nullptr;
+    p->tl_heavy = nullptr;
     p->tl_state = -1;
     const uint64_t V = g->h.V, E = g->h.E;
-    const uint64_t ns = (V + kTlSrcMask) >> kTlSrcBits, nd = (V + kTlDst - 1) >> kTlDstBits;
-    if (g->static_vertices != V || E == 0 || ns > kTlMaxSeg) return ASC_OK;
+    if (db < 6 || db > (uint32_t)kTlMaxDstBits || sb < 8 || sb + db > 32) return ASC_OK;
+    const uint64_t ns = (V + (1ull << sb) - 1) >> sb, nd = (V + (1ull << db) - 1) >> db;
+    if (g->static_vertices != V || E == 0 || ns > kTlMaxSeg || nd > 0xFFFFFFFFull) return ASC_OK;
     size_t free_b = 0, total_b = 0;
     ASC_HIP(hipMemGetInfo(&free_b, &total_b));
     const uint64_t need = 4 * E + 12 * nd * ns + 8 + (V + 31) / 32 * 4;
@@ -764,10 +785,10 @@ static int pr_tiled_build(asc_graph* g, PrState* p) {
     const OffT* off = (const OffT*)g->d_off;
     const uint32_t* src = (const uint32_t*)g->d_edges;
     const uint32_t thr = (uint32_t)p->heavy_thr;
+    const TileShape ts{sb, db, (uint32_t)ns, (uint32_t)nd};
     DevTmp<uint32_t> d_cnt;
     ASC_HIP(hipMalloc(&d_cnt.p, 4 * nd * ns));
-    hipLaunchKernelGGL((k_tl_count<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, (uint32_t)ns,
-                       d_cnt.p);
+    hipLaunchKernelGGL((k_tl_count<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, ts, d_cnt.p);
     ASC_HIP(hipGetLastError());
     std::vector<uint32_t> cnt(nd * ns);
     ASC_HIP(hipMemcpyAsync(cnt.data(), d_cnt.p, 4 * cnt.size(), hipMemcpyDeviceToHost, s));
@@ -784,10 +805,12 @@ static int pr_tiled_build(asc_graph* g, PrState* p) {
     ASC_HIP(hipMalloc(&p->tl_edges, 4 * std::max<unsigned long long>(acc, 1)));
     ASC_HIP(hipMalloc(&p->tl_heavy, 4 * ((V + 31) / 32)));
     ASC_HIP(hipMemsetAsync(p->tl_heavy, 0, 4 * ((V + 31) / 32), s));
-    hipLaunchKernelGGL((k_tl_fill<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, (uint32_t)ns,
+    hipLaunchKernelGGL((k_tl_fill<OffT>), dim3((uint32_t)nd), dim3(kBlock), 0, s, off, src, V, thr, ts,
                        (const unsigned long long*)p->tl_off, p->tl_edges, p->tl_heavy);
     ASC_HIP(hipGetLastError());
     ASC_HIP(hipStreamSynchronize(s));
+    p->tl_sb = sb;
+    p->tl_db = db;
     p->tl_nd = (uint32_t)nd;
     p->tl_ns = (uint32_t)ns;
     p->tl_edges_n = acc;
@@ -817,7 +840,9 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         set_last_error("propagation-blocked PageRank needs every edge resident, E < 2^32 and room for its slots");
         return ASC_ERR_UNSUPPORTED;
     }
-    if (o->method == ASC_PR_TILED) ASC_TRY(pr_tiled_build<OffT>(g, p));
+    const uint32_t sb = o->tile_src_bits > 0 ? (uint32_t)o->tile_src_bits : 18;
+    const uint32_t db = o->tile_dst_bits > 0 ? (uint32_t)o->tile_dst_bits : (uint32_t)kTlMaxDstBits;
+    if (o->method == ASC_PR_TILED) ASC_TRY(pr_tiled_build<OffT>(g, p, sb, db));
     const bool tiled = o->method == ASC_PR_TILED && p->tl_state == 1;
     if (o->method == ASC_PR_TILED && !tiled) {
         set_last_error("source-segmented PageRank needs every edge resident, V <= 2^31 and room for its tiles");
@@ -850,8 +875,11 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         a.v_end = g->static_vertices;
         a.skip_heavy = 1;
         if (tiled) {
-            hipLaunchKernelGGL(k_pr_tiled, dim3(p->tl_nd), dim3(kTlBlock), 0, s, a, (const uint32_t*)p->tl_edges,
-                               (const unsigned long long*)p->tl_off, p->tl_ns,
+            const TileShape ts{p->tl_sb, p->tl_db, p->tl_ns, p->tl_nd};
+            // two 1024-lane workgroups per CU (64 KB of LDS each)
+            const uint32_t tgrid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 2, p->tl_nd);
+            hipLaunchKernelGGL(k_pr_tiled, dim3(tgrid), dim3(kTlBlock), 0, s, a, (const uint32_t*)p->tl_edges,
+                               (const unsigned long long*)p->tl_off, ts,
                                p->n_heavy ? (const uint32_t*)p->tl_heavy : nullptr);
             ASC_HIP(hipGetLastError());
         } else if (blocked) {

@@ -175,6 +175,12 @@ __device__ __forceinline__ void block_add(unsigned long long x, unsigned long lo
 // takes level + 2 (values are level + 1, unreached V + 1).  Each wave owns 64
 // consecutive vertices, i.e. two bitmap words, and writes them whole: `next`
 // needs no clearing and `visited` no atomics (no other wave writes them).
+// A lane scans the first kBuLane edges of its own vertex (most vertices find
+// a parent among them); the rest of any longer adjacency is then scanned by
+// the whole wave, 128 edges per pass with coalesced loads, so a wave no longer
+// waits for one lane walking a long list edge by edge.
+constexpr int kBuLane = 8;
+
 template <class OffT, class EdgeT>
 __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const EdgeT* edges, uint32_t* val,
                                                           uint32_t* visited, const uint32_t* front, uint32_t* next,
@@ -189,15 +195,19 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const
         const uint32_t v = base + lane;
         bool found = false;
         uint64_t deg = 0;
+        OffT b = 0, e = 0;
         if (v < V && !((visited[v >> 5] >> (v & 31)) & 1u)) {
-            const OffT b = off[v], e = off[v + 1];
+            b = off[v];
+            e = off[v + 1];
             deg = (uint64_t)(e - b);
             scanned++;
-            // four loads in flight per lane before the first bitmap probe
-            for (OffT i = b; i < e && !found; i += 4) {
+            // the first kBuLane edges: four loads in flight per lane before
+            // the first bitmap probe
+            const OffT el = deg > (uint64_t)kBuLane ? b + (OffT)kBuLane : e;
+            for (OffT i = b; i < el && !found; i += 4) {
                 uint32_t u[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) u[j] = (i + (OffT)j < e) ? edge_dst(edges[i + j]) : 0xFFFFFFFFu;
+                for (int j = 0; j < 4; j++) u[j] = (i + (OffT)j < el) ? edge_dst(edges[i + j]) : 0xFFFFFFFFu;
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (!found && u[j] != 0xFFFFFFFFu) {
@@ -206,6 +216,27 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const
                     }
                 }
             }
+        }
+        // longer adjacencies not yet settled: one vertex at a time, whole wave
+        uint64_t todo = wave_ballot(!found && deg > (uint64_t)kBuLane);
+        while (todo) {
+            const int l = __builtin_ctzll(todo);
+            todo &= todo - 1;
+            const OffT lb = __shfl(b, l, kWave) + (OffT)kBuLane, le = __shfl(e, l, kWave);
+            bool hit = false;
+            for (OffT i0 = lb; i0 < le; i0 += 2 * kWave) {
+                const OffT i1 = i0 + (OffT)lane, i2 = i1 + (OffT)kWave;
+                const uint32_t u1 = i1 < le ? edge_dst(edges[i1]) : 0xFFFFFFFFu;
+                const uint32_t u2 = i2 < le ? edge_dst(edges[i2]) : 0xFFFFFFFFu;
+                const bool h1 = u1 != 0xFFFFFFFFu && ((front[u1 >> 5] >> (u1 & 31)) & 1u);
+                const bool h2 = u2 != 0xFFFFFFFFu && ((front[u2 >> 5] >> (u2 & 31)) & 1u);
+                examined += (u1 != 0xFFFFFFFFu) + (u2 != 0xFFFFFFFFu);
+                if (wave_ballot(h1 || h2)) {
+                    hit = true;
+                    break;
+                }
+            }
+            if ((int)lane == l) found = hit;
         }
         const uint64_t m = wave_ballot(found);
         if (found) {

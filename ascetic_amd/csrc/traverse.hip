@@ -555,7 +555,7 @@ int trav_ensure(asc_graph* g) {
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
         for (int i = 0; i < kPartBufs; i++) {
-            t->pref_chunk[i] = -1;
+            t->buf_chunk[i] = -1;
             if (g->ctx->cfg.stream_mode != ASC_STREAM_ZEROCOPY)
                 ASC_HIP(hipMalloc(&t->part[i], EB * t->part_edges));
             ASC_HIP(hipEventCreateWithFlags(&t->ev_copy[i], hipEventDisableTiming));
@@ -640,7 +640,7 @@ static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long 
 // the active bytes left to gather and D the bytes copied whole, a round
 // takes about max((D + S) / link, S / gather), since gathered parts also
 // cross the link.
-static void choose_dense_chunks(TravState* t, size_t EB) {
+static void choose_dense_chunks(TravState* t, size_t EB, const std::vector<int>& buf_of) {
     const uint32_t n_ch = (uint32_t)t->ch_v.size() - 1;
     const double L = t->link_bps > 0 ? t->link_bps : 50e9, G = t->gather_bps;
     std::vector<uint32_t> order;
@@ -648,6 +648,10 @@ static void choose_dense_chunks(TravState* t, size_t EB) {
     for (uint32_t c = 0; c < n_ch; c++) {
         t->h_ch_dense[c] = 0;
         if (t->h_ch_count[c] == 0) continue;
+        if (buf_of[c] >= 0) {   // already in a part buffer: read there, no transfer
+            t->h_ch_dense[c] = 1;
+            continue;
+        }
         S += (double)t->h_ch_edges[c] * EB;
         if (t->ch_e[c + 1] - t->ch_e[c] <= t->part_edges) order.push_back(c);
     }
@@ -699,12 +703,15 @@ static const char* tail_w8(const asc_graph* g, uint64_t e) {
 // active vertices, kept in their own list segment, are expanded from it.
 // At the start of a run that may stream, the first chunks of the on-demand
 // region are copied into the part buffers while the resident part is
-// relaxed (the link is idle until the first streaming round).  The first
-// round uses a copy when its dense chunks come in that order (chunk b in part
-// buffer b, the round-robin order), else the buffer is simply overwritten.
+// relaxed (the link is idle until the first streaming round).  A round that
+// finds a chunk it copies whole already in a part buffer (copied ahead, or
+// by an earlier round of the run and not overwritten since) reads it there.
 static int prefetch_chunks(asc_graph* g, TravState* t) {
     asc_ctx* ctx = g->ctx;
-    for (int b = 0; b < kPartBufs; b++) t->pref_chunk[b] = -1;
+    for (int b = 0; b < kPartBufs; b++) {
+        t->buf_chunk[b] = -1;
+        t->buf_counted[b] = false;
+    }
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
     if (!n_ch || ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || ctx->cfg.dense_threshold > 1.0) return ASC_OK;
     if (!g->packed && !g->zc_tail.p) return ASC_OK;
@@ -714,7 +721,7 @@ static int prefetch_chunks(asc_graph* g, TravState* t) {
         ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
         ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
         ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
-        t->pref_chunk[b] = b;
+        t->buf_chunk[b] = b;
     }
     return ASC_OK;
 }
@@ -749,6 +756,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     // active edges per chunk -> which chunks are copied whole
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
     uint32_t n_dense = 0, n_sparse = 0, m = 0;
+    std::vector<int> buf_of(n_ch, -1);   // part buffer holding each chunk whole (-1: none)
     if (n_ch) {
         ASC_HIP(hipMemsetAsync(t->d_ch_edges, 0, 8 * (size_t)n_ch, s));
         ASC_HIP(hipMemsetAsync(t->d_ch_count, 0, 4 * (size_t)n_ch, s));
@@ -758,19 +766,28 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_HIP(hipMemcpyAsync(t->h_ch_edges.p, t->d_ch_edges, 8 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipMemcpyAsync(t->h_ch_count.p, t->d_ch_count, 4 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
+        for (int b = 0; b < kPartBufs; b++)
+            if (t->buf_chunk[b] >= 0 && (uint32_t)t->buf_chunk[b] < n_ch) buf_of[t->buf_chunk[b]] = b;
         if (ctx->cfg.dense_threshold > 0) {
             const double thr = ctx->cfg.dense_threshold;
             for (uint32_t c = 0; c < n_ch; c++) {
                 const uint64_t ce = t->ch_e[c + 1] - t->ch_e[c];
                 t->h_ch_dense[c] = t->h_ch_count[c] > 0 && ce <= t->part_edges &&
-                                   (double)t->h_ch_edges[c] >= thr * (double)ce;
+                                   ((double)t->h_ch_edges[c] >= thr * (double)ce || buf_of[c] >= 0);
             }
         } else {
-            choose_dense_chunks(t, EB);
+            choose_dense_chunks(t, EB, buf_of);
         }
         for (uint32_t c = 0; c < n_ch; c++) {
             if (t->h_ch_dense[c]) n_dense++;
             else n_sparse += t->h_ch_count[c];
+        }
+        if (ctx->cfg.verbose > 2) {   // per-chunk active edges (D: copied whole)
+            printf("round %u chunks:", rc.stream_rounds + 1);
+            for (uint32_t c = 0; c < n_ch; c++)
+                if (t->h_ch_count[c])
+                    printf(" %u:%llu%s", c, (unsigned long long)t->h_ch_edges[c], t->h_ch_dense[c] ? "D" : "");
+            printf("\n");
         }
     }
     if (n_dense) {
@@ -920,17 +937,36 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     } else {
         uint32_t nb = 0;   // device part buffers are used round-robin by both loops
         // dense chunks: whole-chunk DMA from the pinned tail, then the chunk's
-        // list segment expanded with edges addressed from the chunk start
-        for (uint32_t c = 0; c < n_ch && n_dense; c++) {
-            if (!t->h_ch_dense[c]) continue;
-            const int b = (int)(nb++ % kPartBufs);
+        // list segment expanded with edges addressed from the chunk start.
+        // Chunks already in a part buffer go first (no transfer to wait for),
+        // then the others in id order, into buffers in turn.
+        std::vector<uint32_t> dorder;
+        for (uint32_t c = 0; c < n_ch && n_dense; c++)
+            if (t->h_ch_dense[c] && buf_of[c] >= 0) dorder.push_back(c);
+        const size_t n_cached = dorder.size();
+        for (uint32_t c = 0; c < n_ch && n_dense; c++)
+            if (t->h_ch_dense[c] && buf_of[c] < 0) dorder.push_back(c);
+        for (size_t oi = 0; oi < dorder.size(); oi++) {
+            const uint32_t c = dorder[oi];
             const uint64_t e0 = t->ch_e[c], e1 = t->ch_e[c + 1];
-            if (t->pref_chunk[b] != (int)c) {   // not copied ahead at the start of the run
+            int b;
+            if (oi < n_cached) {
+                b = buf_of[c];
+                if (t->buf_counted[b]) rc.reused_edges += e1 - e0;   // copied by an earlier round
+                else dense_edges += e1 - e0;                          // copied ahead for this round
+            } else {
+                // the chunks read in place were all launched already, so any
+                // buffer may be overwritten once its last reader is done
+                b = (int)(nb++ % kPartBufs);
                 ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
                 ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
                 ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
+                dense_edges += e1 - e0;
+                if (t->buf_chunk[b] >= 0) buf_of[t->buf_chunk[b]] = -1;
+                t->buf_chunk[b] = (int)c;
+                buf_of[c] = b;
             }
-            t->pref_chunk[b] = -1;
+            t->buf_counted[b] = true;
             ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
             p.zc = (const EdgeT*)t->part[b];
             p.zc32 = (const uint32_t*)t->part[b];
@@ -941,7 +977,6 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             if (p.hi > p.lo) ASC_TRY(launch_part(true));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
-            dense_edges += e1 - e0;
         }
         if (ctx->cfg.stream_mode == ASC_STREAM_HYBRID) {
             // the rest read by the kernel from the mapped pinned tail
@@ -972,6 +1007,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p, stage_w);
             fill_round += now_ms() - tg;
             ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
+            t->buf_chunk[b] = -1;                                       // now holds gathered records
             ASC_TRY(copy_to_part(g, t, b, t->stage[sb].p, stage_w, hi - lo, ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_stage[sb], ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
@@ -993,7 +1029,6 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         if (Ts * EB >= (32u << 20) && fill_round > 0)   // running estimate of the gather rate
             t->gather_bps = 0.5 * t->gather_bps + 0.5 * (double)(Ts * EB) / (fill_round * 1e-3);
     }
-    for (int b = 0; b < kPartBufs; b++) t->pref_chunk[b] = -1;   // buffers reused from here on
     rc.overload_ms += now_ms() - t_begin;
     if (ctx->cfg.verbose > 1)
         printf("stream round %u at %.3f ms: pile %u, %u dense chunks %llu edges, gathered %u vertices %llu edges, "
@@ -1216,6 +1251,7 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
         st->streamed_edges = rc.streamed_edges;
         st->streamed_bytes = rc.streamed_edges * g->stream_record_bytes();
         st->dense_edges = rc.dense_edges;
+        st->reused_edges = rc.reused_edges;
         st->vertices_processed = ds.verts_local + ds.verts_bu;
         st->host_cores = hardware_threads();
         st->path = path;

@@ -108,6 +108,7 @@ struct PartArgs {
 struct RunCounters {
     double pre_ms = 0, overload_ms = 0, fill_ms = 0;
     uint64_t node_sum = 0, streamed_edges = 0, dense_edges = 0;
+    uint64_t reused_edges = 0;      // edges of chunks read again from a part buffer (no copy)
     uint32_t parts = 0, stream_rounds = 0, splits = 0, dense_chunks = 0;
     double t_start = 0;             // host clock at the start of the run (per-round log)
 };
@@ -133,7 +134,12 @@ struct TravState {
     Piles* piles = nullptr;
     DevStats* stats = nullptr;
     void* part[kPartBufs] = {};     // device part buffers, used round-robin
-    int pref_chunk[kPartBufs] = {};  // chunk copied ahead into each part buffer (-1: none)
+    // On-demand chunk whose edges part buffer b holds whole (-1: none), copied
+    // ahead at the start of the run or by an earlier round of it; a round
+    // that needs the chunk reads it there instead of copying it again.
+    // buf_counted[b]: that copy already counts in the run's streamed edges.
+    int buf_chunk[kPartBufs] = {};
+    bool buf_counted[kPartBufs] = {};
     uint64_t part_edges = 0;
     HostBuf<Slot> h_slot;
     HostBuf<Piles> h_piles;

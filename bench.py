@@ -41,6 +41,83 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def default_cpu_threads():
+    """Threads for the CPU baseline: every core this process may run on, up to
+    the 16-core share one GPU gets on the benchmark host."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def oracle_run(label, fn, nt):
+    """fn(nthreads) -> (values, stats) of a restated traversal, timed."""
+    t0 = time.perf_counter()
+    out, st = fn(nt)
+    dt = time.perf_counter() - t0
+    log(f"cpu {label}: {dt:.2f} s on {nt} threads")
+    return out, st, dt
+
+
+def single_thread_sample(O, label, fn, budget_s=4.0):
+    """A bounded single-thread sample of a restated traversal fn(nthreads):
+    the first k iterations (k = 1, 4, 16, ...) until one sample takes at least
+    a quarter of budget_s or the traversal ends.  Returns the rate in edges
+    relaxed per second and what was sampled."""
+    k = 1
+    while True:
+        with O.iteration_limit(k):
+            t0 = time.perf_counter()
+            _, st = fn(1)
+            dt = time.perf_counter() - t0
+        whole = st["iterations"] < k
+        if whole or dt >= budget_s / 4 or k >= 1 << 12:
+            log(f"cpu {label} single thread: {st['iterations']} iterations, {dt:.2f} s")
+            return {"rate": st["edges_relaxed"] / max(dt, 1e-9), "iterations": st["iterations"],
+                    "seconds": round(dt, 3), "whole_run": whole}
+        k *= 4
+
+
+def cpu_baseline_block(gteps, nt, st_all, t_all, single, what):
+    """cpu_baseline entry: all-core GTEPS of the full restated run, and a
+    single-thread figure in edges relaxed per second (with the GTEPS a single
+    thread would reach relaxing the same edges at that rate)."""
+    rate_all = st_all["edges_relaxed"] / max(t_all, 1e-9)
+    return {
+        "value": round(gteps, 4), "unit": "GTEPS", "cores": nt, "kind": "port",
+        "sample": f"{what}, the whole run on {nt} threads ({t_all:.2f} s)",
+        "edges_relaxed_per_s": round(rate_all / 1e9, 4),
+        "single_thread": {
+            "value": round(single["rate"] / 1e9, 4), "unit": "G edges relaxed/s", "cores": 1,
+            "gteps_equivalent": round(gteps * single["rate"] / max(rate_all, 1e-9), 4),
+            "sample": f"first {single['iterations']} iterations ({'whole run' if single['whole_run'] else 'bounded'}), "
+                      f"{single['seconds']} s"},
+    }
+
+
+def gather_baseline(O, csr, ew, vs, V, overload_edges, _unused=None):
+    """The reference's 20-thread host gather (GraphMeta.cuh:469-514) timed
+    alone: every on-demand vertex with edges, in id order, copied as 8-byte
+    {dst, w} records (median of three copies)."""
+    deg = np.diff(csr.off)
+    lst = np.arange(vs, V, dtype=np.uint32)
+    lst = lst[deg[vs:] > 0]
+    times = []
+    for _ in range(3):
+        buf, ms = O.gather_ref(csr, lst, 20, aos=ew)
+        times.append(ms)
+        del buf
+    ms = float(np.median(times))
+    nbytes = 8 * int(deg[vs:].sum())
+    rate = nbytes / (ms / 1e3)
+    log(f"cpu reference gather: {nbytes / 1e9:.2f} GB in {ms:.1f} ms on 20 threads")
+    return {"threads": 20, "bytes": nbytes, "ms": round(ms, 2), "GBps": round(rate / 1e9, 2),
+            "per_run_ms": round(8 * overload_edges / rate * 1e3, 2),
+            "sample": "the whole on-demand region gathered once; per_run_ms = the restated run's gathered edges "
+                      "at that rate"}
+
+
 def algorithmic_bytes(kern, s_e, s_o):
     """Bytes a work-optimal push relax must move: per edge the record and one
     destination value; per frontier vertex an offset pair, its value, a queue
@@ -52,12 +129,15 @@ def algorithmic_bytes(kern, s_e, s_o):
     }
 
 
-def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None, step_graphs=True):
+def serving_bench(A, O, scale, seed, n_queries, rate_factors, cpu_threads, budget_s=60.0, h=None,
+                  step_graphs=True):
     """Query serving on a resident weighted R-MAT graph: 70% BFS / 30% SSSP,
     sources uniform over vertices with degree > 0.  Reports one-query-per-
     launch throughput (plain kernels), the server's saturated throughput and
     Poisson-arrival sweeps with p50/p99 submit-to-done latency, and checks a
-    sample of served results against the same queries run alone."""
+    sample of served results against the same queries run alone and, for
+    four of them, against the oracle; the oracle's restatement answering the
+    same kind of queries on the host cores is the CPU baseline."""
     import threading
 
     t = time.perf_counter()
@@ -71,6 +151,8 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None
     srcs = [int(x) for x in rng.choice(cand, n_queries)]
     kinds = [A.Server.SSSP if x < 0.3 else A.Server.BFS for x in rng.random(n_queries)]
     out = {"workload": f"rmat_s{scale}_ef16_w1-64_resident_70bfs_30sssp", "queries": n_queries}
+    # views of the host CSR, which the graph keeps (and frees when closed)
+    g_host_arrays = (h.offsets(), h.edge_words())
     with A.Context() as ctx:
         g = ctx.adopt(h, offset_bytes=8)
         # one query per launch
@@ -96,11 +178,42 @@ def serving_bench(A, scale, seed, n_queries, rate_factors, budget_s=60.0, h=None
             sat = n_queries / (time.perf_counter() - t0)
             exact = all(np.array_equal(bufs[i], (g.sssp if kinds[i] == A.Server.SSSP else g.bfs)(srcs[i])[0])
                         for i in sample)
+            # the oracle on two sampled queries of each kind, which also times
+            # the CPU answering them (all cores, then one thread, bounded)
+            off, ew = g_host_arrays
+            csr = O.CSR(off, ew[:, 0], ew[:, 1])
+            picks = [i for i in sample if kinds[i] == A.Server.BFS][:2] + \
+                    [i for i in sample if kinds[i] == A.Server.SSSP][:2]
+            cpu_t = {A.Server.BFS: [], A.Server.SSSP: []}
+            oracle_ok = True
+            for i in picks:
+                fn = (lambda nt, s0=srcs[i]: O.sssp_ref(csr, s0, nthreads=nt)) if kinds[i] == A.Server.SSSP else \
+                    (lambda nt, s0=srcs[i]: O.bfs_ref(csr, s0, nthreads=nt))
+                ref, _, dt = oracle_run(f"served query {i}", fn, cpu_threads)
+                oracle_ok = oracle_ok and bool(np.array_equal(bufs[i], ref))
+                cpu_t[kinds[i]].append(dt)
+                del ref
             out["saturated_qps"] = round(sat, 2)
             out["saturated_p50_ms"] = round(float(np.percentile(lat, 50)), 3)
             out["saturated_p99_ms"] = round(float(np.percentile(lat, 99)), 3)
             out["speedup_vs_plain"] = round(sat / plain_qps, 2)
             out["exact_sample"] = f"{len(sample)} queries bit-identical to running alone" if exact else "MISMATCH"
+            out["oracle_sample"] = (f"{len(picks)} served queries bit-identical to the oracle" if oracle_ok
+                                    else "MISMATCH vs oracle")
+            tb = float(np.mean(cpu_t[A.Server.BFS])) if cpu_t[A.Server.BFS] else 0.0
+            ts = float(np.mean(cpu_t[A.Server.SSSP])) if cpu_t[A.Server.SSSP] else 0.0
+            if tb > 0 and ts > 0:
+                # single thread: the first BFS query, bounded
+                b0 = next(i for i in picks if kinds[i] == A.Server.BFS)
+                single = single_thread_sample(O, "served bfs", lambda nt: O.bfs_ref(csr, srcs[b0], nthreads=nt))
+                out["cpu_baseline"] = {
+                    "value": round(1.0 / (0.7 * tb + 0.3 * ts), 3), "unit": "queries/s", "cores": cpu_threads,
+                    "kind": "port",
+                    "sample": f"oracle bfs_ref / sssp_ref answering {len(picks)} of the sampled queries one at a time "
+                              f"on {cpu_threads} threads (BFS {tb:.2f} s, SSSP {ts:.2f} s each), weighted 70/30",
+                    "single_thread": {"value": round(single["rate"] / 1e9, 4), "unit": "G edges relaxed/s",
+                                      "cores": 1, "sample": f"first {single['iterations']} iterations of one BFS "
+                                                            f"query, {single['seconds']} s"}}
             sweeps = []
             for f in rate_factors:
                 rate = f * plain_qps
@@ -168,17 +281,22 @@ def _kernel_table(st, s_e, s_o, V):
     return out
 
 
-def resident_bench(A, _abi, h, steps):
+def resident_bench(A, _abi, O, h, steps, cpu_threads):
     """BFS, SSSP and CC on one fully resident weighted R-MAT graph (u64
     offsets, {dst, w} records).  BFS runs direction-optimizing and CC by
-    union-find (the graph is symmetric); both are compared bit for bit with
-    the library's top-down BFS and label-propagation CC on the same device."""
+    union-find (the graph is symmetric); every result is compared bit for
+    bit with the oracle's restatement of the reference at full size, which
+    is also timed on all host cores and on one thread (CPU baseline)."""
     off, ew = h.offsets(), h.edge_words()
     V, E = h.num_vertices, h.num_edges
     deg = np.diff(off)
     src = int(np.argmax(deg))
     out = {"V": V, "E_directed": E, "source": src}
     s_e, s_o = 8, 8
+    csr = O.CSR(off, ew[:, 0], ew[:, 1])
+    oracles = {"bfs": (lambda nt: O.bfs_ref(csr, src, nthreads=nt), "bfs_opt"),
+               "sssp": (lambda nt: O.sssp_ref(csr, src, nthreads=nt), "sssp_opt"),
+               "cc": (lambda nt: O.cc_ref(csr, nthreads=nt), "cc_opt")}
     with A.Context(profile=True) as ctx, ctx.from_arrays(off, ew) as g:
         out["symmetric"] = g.info()["symmetric"]
         for algo in ("bfs", "sssp", "cc"):
@@ -191,22 +309,33 @@ def resident_bench(A, _abi, h, steps):
             out[algo] = {"gteps": round(num / (ms / 1e3) / 1e9, 3), "ms": round(ms, 3), "path": st["path"],
                          "iterations": st["iterations"], "reached": st["reached"],
                          "kernels": _kernel_table(runs[len(runs) // 2], s_e, s_o, V)}
+            ofn, what = oracles[algo]
+            ref, cst, tcpu = oracle_run(f"{algo} resident", ofn, cpu_threads)
+            out[algo]["exact"] = bool(np.array_equal(first, ref))
+            out[algo]["parity"] = f"oracle {algo}_ref (the reference's {what} restated), full size"
+            single = single_thread_sample(O, f"{algo} resident", ofn)
+            out[algo]["cpu_baseline"] = cpu_baseline_block(num / tcpu / 1e9, cpu_threads, cst, tcpu, single,
+                                                           f"oracle restatement of {what}")
+            del ref
             if algo in ("bfs", "cc"):
+                # the reference's own schedules (top-down frontier BFS, label
+                # propagation) on the same device, for comparison
                 with A.Context(profile=True, bfs_direction=_abi.BFS_TOP_DOWN, cc_method=_abi.CC_LABEL_PROP) as c2, \
                         c2.from_arrays(off, ew) as g2:
-                    ref, st2 = getattr(g2, algo)(*args)
-                    plain = [getattr(g2, algo)(*args, output=False)[1]["total_ms"] for _ in range(max(1, steps // 2))]
-                out[algo]["exact_vs_plain"] = bool(np.array_equal(first, ref))
-                out[algo]["plain_ms"] = round(float(np.median(plain)), 3)
-                out[algo]["plain_path"] = st2["path"]
-                del ref
+                    plain = [getattr(g2, algo)(*args, output=False)[1] for _ in range(max(1, steps // 2))]
+                out[algo]["plain_ms"] = round(float(np.median([p["total_ms"] for p in plain])), 3)
+                out[algo]["plain_path"] = plain[0]["path"]
             del first
     return out
 
 
-def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads):
+def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads, pr_method):
     """GTEPS per algorithm: BFS on R-MAT (C1 shape) and CC / PageRank on an
-    Erdos-Renyi graph (C3 shape), fully resident."""
+    Erdos-Renyi graph (C3 shape), fully resident.  Every result is compared
+    with the oracle at full size (BFS levels and CC labels bit for bit,
+    PageRank by the L1 distance to the FP64 power iteration), and the
+    oracle's restatement of the reference is timed on all host cores and on
+    one thread as the CPU baseline."""
     out = {}
     # BFS: R-MAT scale 20, ef 16, symmetrized, u32 offsets, source 0
     t = time.perf_counter()
@@ -218,18 +347,19 @@ def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads
         lv, st = g.bfs(0)
         runs = [g.bfs(0, output=False)[1] for _ in range(steps)]
         ms = float(np.median([r["total_ms"] for r in runs]))
-        gteps = st["reached_degree_sum"] / 2 / (ms / 1e3) / 1e9
+        num = st["reached_degree_sum"] / 2
+        gteps = num / (ms / 1e3) / 1e9
         csr = O.CSR(off, dst)
-        tc = time.perf_counter()
-        ref, _ = O.bfs_ref(csr, 0, nthreads=cpu_threads)
-        tcpu = time.perf_counter() - tc
+        ref, cst, tcpu = oracle_run("bfs C1", lambda nt: O.bfs_ref(csr, 0, nthreads=nt), cpu_threads)
+        single = single_thread_sample(O, "bfs C1", lambda nt: O.bfs_ref(csr, 0, nthreads=nt))
         out["bfs"] = {"workload": f"bfs_rmat_s{bfs_scale}_ef16_src0_u32off", "gteps": round(gteps, 3),
                       "ms": round(ms, 3), "levels": st["iterations"], "reached": st["reached"],
-                      "exact": bool(np.array_equal(lv, ref)), "path": st["path"],
-                      "kernels": _kernel_table(runs[len(runs) // 2], 4, 4, h.num_vertices),
-                      "cpu_port_gteps": round(st["reached_degree_sum"] / 2 / tcpu / 1e9, 3), "cpu_cores": cpu_threads}
+                      "exact": bool(np.array_equal(lv, ref)), "parity": "oracle bfs_ref, full size",
+                      "path": st["path"], "kernels": _kernel_table(runs[len(runs) // 2], 4, 4, h.num_vertices),
+                      "cpu_baseline": cpu_baseline_block(num / tcpu / 1e9, cpu_threads, cst, tcpu, single,
+                                                         "oracle restatement of bfs_opt")}
         g.close()
-    del h, off, dst
+    del h, off, dst, ref
     # CC and PageRank: Erdos-Renyi, 2^er_log_v vertices, 2^er_log_e undirected
     # pairs, symmetrized (so the out-CSR is also the in-CSR PageRank pulls over)
     t = time.perf_counter()
@@ -237,6 +367,7 @@ def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads
     V, E = h.num_vertices, h.num_edges
     off, dst = h.offsets(), h.dst()
     log(f"er graph: V={V} E={E} ({time.perf_counter() - t:.1f}s)")
+    csr = O.CSR(off, dst)
     with A.Context(profile=True) as ctx:
         with ctx.from_arrays(off, dst) as g:
             lab, st = g.cc()
@@ -246,20 +377,51 @@ def per_algo_bench(A, _abi, O, bfs_scale, er_log_v, er_log_e, steps, cpu_threads
                          "ms": round(ms, 3), "iterations": st["iterations"], "path": st["path"],
                          "components": int(np.count_nonzero(lab == np.arange(V, dtype=np.uint32))),
                          "kernels": _kernel_table(ccs[len(ccs) // 2], 4, 8, V)}
-        # label propagation on the same graph: the labels must be identical
+        # the oracle's label propagation (cc_opt restated), all cores: labels
+        # must be identical; it is also the CPU baseline
+        ref, cst, tcpu = oracle_run("cc C3", lambda nt: O.cc_ref(csr, nthreads=nt), cpu_threads)
+        out["cc"]["exact"] = bool(np.array_equal(lab, ref))
+        out["cc"]["parity"] = "oracle cc_ref (label propagation restated), full size"
+        single = single_thread_sample(O, "cc C3", lambda nt: O.cc_ref(csr, nthreads=nt))
+        out["cc"]["cpu_baseline"] = cpu_baseline_block(E / 2 / tcpu / 1e9, cpu_threads, cst, tcpu, single,
+                                                       "oracle restatement of cc_opt")
         with A.Context(profile=True, cc_method=_abi.CC_LABEL_PROP) as c2, c2.from_arrays(off, dst) as g2:
-            ref, st2 = g2.cc()
-            out["cc"]["exact_vs_plain"] = bool(np.array_equal(lab, ref))
-            out["cc"]["plain_ms"] = round(st2["total_ms"], 3)
+            _, st2 = g2.cc(output=False)
+            out["cc"]["label_propagation_ms"] = round(st2["total_ms"], 3)
         del lab, ref
         outdeg = np.diff(off).astype(np.uint32)
         with ctx.from_arrays(off, dst, out_degree=outdeg) as g:
-            _, res, _ = g.pagerank(iterations=20)
-            prs = [g.pagerank(iterations=20)[2] for _ in range(max(1, steps // 2))]
+            r, res, _ = g.pagerank(iterations=20, method=pr_method)
+            prs = [g.pagerank(iterations=20, method=pr_method)[2] for _ in range(max(1, steps // 2))]
             ms = float(np.median([p["total_ms"] for p in prs]))
             out["pagerank"] = {"workload": f"pr_er_v2^{er_log_v}_e2^{er_log_e}_20it_fp32",
                                "gteps": round(E * 20 / (ms / 1e3) / 1e9, 3), "ms": round(ms, 3),
-                               "final_residual": float(res[-1]), "pr_pull_GBps": _kernel_rate(prs[-1], "pr_pull", 4, 8)}
+                               "method": {_abi.PR_AUTO: "auto", _abi.PR_PULL: "pull", _abi.PR_TILED: "tiled",
+                                          _abi.PR_BLOCKED: "blocked"}[pr_method],
+                               "final_residual": float(res[-1]),
+                               "kernel_GBps": _kernel_rate(prs[-1], "pr_pull", 4, 8)}
+        # FP64 power iteration of the same formula on all cores: L1 distance
+        O.set_num_threads(cpu_threads)
+        t0 = time.perf_counter()
+        ref, _ = O.pr_norm(csr, outdeg, 0.85, 20)
+        tcpu = time.perf_counter() - t0
+        l1 = float(np.abs(r.astype(np.float64) - ref).sum())
+        log(f"cpu pagerank C3: {tcpu:.2f} s on {cpu_threads} threads, L1 {l1:.3g}")
+        out["pagerank"]["l1_vs_fp64_oracle"] = l1
+        out["pagerank"]["parity"] = ("L1 <= 1e-5 vs oracle pr_norm (FP64), full size" if l1 <= 1e-5
+                                     else "MISMATCH: L1 above 1e-5")
+        O.set_num_threads(1)
+        t0 = time.perf_counter()
+        O.pr_norm(csr, outdeg, 0.85, 1)
+        t1 = time.perf_counter() - t0
+        O.set_num_threads(cpu_threads)
+        log(f"cpu pagerank C3 single thread: one iteration {t1:.2f} s")
+        out["pagerank"]["cpu_baseline"] = {
+            "value": round(E * 20 / tcpu / 1e9, 4), "unit": "GTEPS", "cores": cpu_threads, "kind": "port",
+            "sample": f"oracle FP64 power iteration, 20 iterations on {cpu_threads} threads ({tcpu:.2f} s)",
+            "single_thread": {"value": round(E / t1 / 1e9, 4), "unit": "GTEPS", "cores": 1,
+                              "sample": f"one iteration of the same ({t1:.2f} s)"}}
+        del ref, r
     h.free()
     return out
 
@@ -400,20 +562,23 @@ def stream_section(A, _abi, args):
 
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        nt = args.cpu_threads or min(16, os.cpu_count() or 1)
+        nt = args.cpu_threads or default_cpu_threads()
         csr = O.CSR(off, ew[:, 0], ew[:, 1])
-        tc = time.perf_counter()
-        ref, cst = O.sssp_ref(csr, src, info["static_edges"], info["part_edges"], nt)
-        tcpu = time.perf_counter() - tc
+        S, PE = info["static_edges"], info["part_edges"]
+        ref, cst, tcpu = oracle_run("sssp streamed", lambda n: O.sssp_ref(csr, src, S, PE, n), nt)
         ok = bool(np.array_equal(ref, dist))
-        result["cpu_baseline"] = {
-            "value": round((reached_deg / 2) / tcpu / 1e9, 4),
-            "unit": "GTEPS",
-            "cores": nt,
-            "kind": "port",
-            "sample": f"one full SSSP of the same graph and source by the oracle's restatement of "
-                      f"CalculateOpt.cu sssp_opt (same static prefix and part size), {tcpu:.2f} s",
-        }
+        del ref
+        single = single_thread_sample(O, "sssp streamed", lambda n: O.sssp_ref(csr, src, S, PE, n))
+        result["cpu_baseline"] = cpu_baseline_block(
+            (reached_deg / 2) / tcpu / 1e9, nt, cst, tcpu, single,
+            "one SSSP of the same graph and source by the oracle's restatement of CalculateOpt.cu sssp_opt "
+            "(same static prefix and part size)")
+        result["cpu_baseline"]["host_cores_visible"] = os.cpu_count()
+        # the reference's host gather (fillEdgeArrByMultiThread, 20 threads)
+        # timed alone over the whole on-demand region, and the time it would
+        # spend on the edges the restated run gathered
+        result["cpu_baseline"]["reference_gather"] = gather_baseline(O, csr, ew, int(info["static_vertices"]), V,
+                                                                     cst["overload_edges"], args.steps and 1)
         result["parity"] = "bit-exact vs oracle" if ok else "MISMATCH vs oracle"
         if not ok:
             log("GPU distances differ from the oracle")
@@ -421,6 +586,72 @@ def stream_section(A, _abi, args):
     ctx.close()
     del h, off, ew
     return result
+
+
+def c4_shape_section(A, _abi, O, args, cpu_threads):
+    """SSSP and BFS at the shape of the out-of-memory configuration (R-MAT
+    edge factor 32, weights uniform in [1, 255], about a third of the edge
+    bytes resident, the rest streamed from pinned host memory; source = the
+    highest-degree vertex) at a scale one benchmark run can generate and
+    hold.  Ids then need more bits than fit beside an 8-bit weight in one
+    32-bit word, so on-demand edges stream as 5-byte split records (u32 id +
+    weight byte), as they would at scale 30.  Reports GTEPS, link GB/s against
+    the measured pinned peak, oracle parity and the CPU baseline."""
+    t = time.perf_counter()
+    sc, ef = args.c4_scale, args.c4_edge_factor
+    h = A.rmat(sc, ef, seed=args.seed, weighted=True, wmin=1, wmax=255)
+    off, ew = h.offsets(), h.edge_words()          # views; the graph keeps the arrays
+    V, E = h.num_vertices, h.num_edges
+    deg = np.diff(off)
+    src = int(np.argmax(deg))
+    log(f"c4-shape graph: R-MAT s{sc} ef{ef} w1-255 V={V} E={E} ({time.perf_counter() - t:.1f}s)")
+    budget = int(args.c4_budget_frac * 8 * E)
+    out = {"workload": f"rmat_s{sc}_ef{ef}_w1-255_budget{int(round(args.c4_budget_frac * 100))}",
+           "graph": f"R-MAT scale {sc}, edge factor {ef}, (0.57,0.19,0.19,0.05), symmetrized, weights uniform in "
+                    f"[1, 255], seed {args.seed} (the scale-30 configuration's shape at the largest scale one "
+                    f"benchmark run generates and holds)",
+           "V": V, "E_directed": E, "source": src, "edge_budget_bytes": budget}
+    with A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, profile=True) as ctx:
+        g = ctx.adopt(h, offset_bytes=8)
+        info = g.info()
+        out.update({"static_edges": info["static_edges"], "ondemand_edges": info["ondemand_edges"]})
+        h2d, _ = ctx.measure_link(256 << 20, 8)
+        csr = O.CSR(off, ew[:, 0], ew[:, 1])
+        S, PE = info["static_edges"], info["part_edges"]
+        n_runs = max(2, args.steps // 4)
+        for algo in ("sssp", "bfs"):
+            fn = getattr(g, algo)
+            first, st = fn(src)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            runs = [fn(src, output=False)[1] for _ in range(n_runs)]
+            ctx.synchronize()
+            step_s = (time.perf_counter() - t0) / n_runs
+            num = st["reached_degree_sum"] / 2
+            sb = float(np.mean([r["streamed_bytes"] for r in runs]))
+            sec = {"gteps": round(num / step_s / 1e9, 4), "ms_per_run": round(step_s * 1e3, 3), "runs": n_runs,
+                   "iterations": st["iterations"], "reached": st["reached"],
+                   "stream_record_bytes": round(sb / max(1.0, float(np.mean([r["streamed_edges"] for r in runs])))),
+                   "link": {"streamed_GB_per_run": round(sb / 1e9, 3), "achieved_GBps": round(sb / step_s / 1e9, 2),
+                            "peak_h2d_GBps": round(h2d, 2), "frac": round(sb / step_s / 1e9 / h2d, 4) if h2d else None,
+                            "gather_ms_per_run": round(float(np.mean([r["fill_ms"] for r in runs])), 2)},
+                   "kernels": _kernel_table(runs[len(runs) // 2], 8, 8, V)}
+            ofn = (lambda nt: O.sssp_ref(csr, src, S, PE, nt)) if algo == "sssp" else \
+                (lambda nt: O.bfs_ref(csr, src, S, PE, nt))
+            ref, cst, tcpu = oracle_run(f"{algo} c4-shape", ofn, cpu_threads)
+            sec["exact"] = bool(np.array_equal(first, ref))
+            sec["parity"] = f"oracle {algo}_ref (same static prefix and part size), full size"
+            del ref, first
+            single = single_thread_sample(O, f"{algo} c4-shape", ofn)
+            sec["cpu_baseline"] = cpu_baseline_block(num / tcpu / 1e9, cpu_threads, cst, tcpu, single,
+                                                     f"oracle restatement of {algo}_opt (same static prefix)")
+            if algo == "sssp":
+                sec["cpu_baseline"]["reference_gather"] = gather_baseline(O, csr, ew, int(info["static_vertices"]),
+                                                                          V, cst["overload_edges"])
+            out[algo] = sec
+            log(f"c4-shape {algo}: {sec['gteps']} GTEPS, {sec['ms_per_run']} ms/run, exact {sec['exact']}")
+        g.close()
+    return out
 
 
 def main():
@@ -442,7 +673,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-stream-baselines", dest="stream_baselines", action="store_false",
                     help="skip the partition-streaming and gather-only comparison runs")
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="CPU baseline threads (0 = the cores this process may use, at most 16)")
     ap.add_argument("--serve-scale", type=int, default=26, help="R-MAT scale of the serving section (0 = skip)")
     ap.add_argument("--algo-bfs-scale", type=int, default=20, help="R-MAT scale of the per-algo BFS line (0 = skip)")
     ap.add_argument("--algo-er", type=str, default="26,30",
@@ -454,6 +686,13 @@ def main():
                     help="R-MAT scale of the resident BFS/SSSP/CC section (0 = skip)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also launch one plain device copy of this many MiB (counter calibration)")
+    ap.add_argument("--pr-method", choices=["auto", "pull", "tiled", "blocked"], default="auto",
+                    help="PageRank schedule of the per-algorithm line")
+    ap.add_argument("--c4-scale", type=int, default=26,
+                    help="R-MAT scale of the section at the out-of-memory configuration's shape (0 = skip)")
+    ap.add_argument("--c4-edge-factor", type=int, default=32)
+    ap.add_argument("--c4-budget-frac", type=float, default=0.35,
+                    help="device edge budget of that section as a fraction of its edge+weight bytes")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -463,7 +702,9 @@ def main():
 
     import ascetic_amd as A
     from ascetic_amd import _abi
+    from oracle import oracle as O
 
+    cpu_threads = args.cpu_threads or default_cpu_threads()
     if args.scale > 0:
         result = stream_section(A, _abi, args)
     else:   # sections only (profiling one workload at a time)
@@ -471,10 +712,10 @@ def main():
                   "warmup": args.warmup, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                   "dtype": "u32", "data": "synthetic", "config": {"workload": "sections only"}}
     if args.algo_bfs_scale > 0 and args.algo_er:
-        from oracle import oracle as O
         lv, le = (int(x) for x in args.algo_er.split(","))
-        result["per_algo"] = per_algo_bench(A, _abi, O, args.algo_bfs_scale, lv, le, args.steps,
-                                            args.cpu_threads or min(16, os.cpu_count() or 1))
+        pm = {"auto": _abi.PR_AUTO, "pull": _abi.PR_PULL, "tiled": _abi.PR_TILED, "blocked": _abi.PR_BLOCKED}
+        result["per_algo"] = per_algo_bench(A, _abi, O, args.algo_bfs_scale, lv, le, args.steps, cpu_threads,
+                                            pm[args.pr_method])
         if args.scale > 0:
             result["per_algo"]["sssp"] = {"workload": result["config"]["workload"], "gteps": result["value"]}
     hr = None
@@ -484,18 +725,44 @@ def main():
         log(f"resident graph: R-MAT s{args.resident_scale} ef16 weighted V={hr.num_vertices} E={hr.num_edges} "
             f"({time.perf_counter() - t:.1f}s)")
         result["resident"] = {"workload": f"rmat_s{args.resident_scale}_ef16_w1-64_resident_u64off",
-                              **resident_bench(A, _abi, hr, args.steps)}
+                              **resident_bench(A, _abi, O, hr, args.steps, cpu_threads)}
         if args.serve_scale != args.resident_scale:
             hr.free()
             hr = None
     if args.serve_scale > 0:
         # the serving section adopts (consumes) the resident graph when the scales match
-        result["serving"] = serving_bench(A, args.serve_scale, args.seed, args.serve_queries, [0.25, 1.0, 4.0, 8.0],
-                                          h=hr, step_graphs=not args.serve_eager)
+        result["serving"] = serving_bench(A, O, args.serve_scale, args.seed, args.serve_queries,
+                                          [0.25, 1.0, 4.0, 8.0], cpu_threads, h=hr, step_graphs=not args.serve_eager)
         result["serving"]["step_graphs"] = not args.serve_eager
+        hr = None
+    if args.c4_scale > 0:
+        result["c4_shape"] = c4_shape_section(A, _abi, O, args, cpu_threads)
     print(json.dumps(result), flush=True)
-    if result.get("parity", "").startswith("MISMATCH"):
+    bad = [k for k, v in _parity_flags(result) if not v]
+    if bad:
+        log("parity failures: " + ", ".join(bad))
         sys.exit(1)
+
+
+def _parity_flags(result):
+    """(section, ok) for every parity check in the JSON line."""
+    flags = []
+    if "parity" in result:
+        flags.append(("main", not result["parity"].startswith("MISMATCH")))
+    for k, v in result.get("per_algo", {}).items():
+        if "exact" in v:
+            flags.append((f"per_algo.{k}", v["exact"]))
+        if k == "pagerank":
+            flags.append(("per_algo.pagerank", not v.get("parity", "").startswith("MISMATCH")))
+    for sec in ("resident", "c4_shape"):
+        for k, v in result.get(sec, {}).items():
+            if isinstance(v, dict) and "exact" in v:
+                flags.append((f"{sec}.{k}", v["exact"]))
+    sv = result.get("serving", {})
+    if sv:
+        flags.append(("serving.exact_sample", not sv.get("exact_sample", "").startswith("MISMATCH")))
+        flags.append(("serving.oracle_sample", not sv.get("oracle_sample", "").startswith("MISMATCH")))
+    return flags
 
 
 if __name__ == "__main__":

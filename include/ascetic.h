@@ -277,6 +277,9 @@ typedef struct {
      * [2] on-demand parts, [3] PageRank pull, [4] BFS bottom-up steps,
      * [5] union-find link passes. */
     asc_kernel_stats kern[6];
+    uint64_t reused_edges;       /* on-demand edges read again from a chunk an
+                                    earlier round of the run left in a part
+                                    buffer (no transfer; not in streamed_edges) */
 } asc_stats;
 
 /* ---- algorithms (CalculateOpt.cu:14-721) ------------------------------- */
@@ -298,6 +301,10 @@ typedef struct {
     double threshold;            /* reference mode: 0.001                     */
     int max_iterations;          /* reference mode safety cap (0 = 10000)     */
     int method;                  /* normalised mode: asc_pr_method            */
+    /* ASC_PR_TILED tile shape (0 = tuned): sources per segment 2^src_bits,
+     * destinations per block 2^dst_bits (<= 14), src_bits + dst_bits <= 32 */
+    int tile_src_bits;
+    int tile_dst_bits;
 } asc_pr_opts;
 
 /* Normalised PageRank schedule (asc_pr_opts.method). */
