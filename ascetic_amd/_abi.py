@@ -57,7 +57,7 @@ class Config(C.Structure):
         ("cc_method", C.c_int),
         ("dense_threshold", C.c_double),
         ("stream_overlap", C.c_int),
-        ("pad_cfg", C.c_int),
+        ("bfs_host_stepped", C.c_int),
     ]
 
 
@@ -102,7 +102,7 @@ class KernelStats(C.Structure):
     ]
 
 
-KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull", "bottom_up", "union_find")
+KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull", "bottom_up", "union_find", "bfs_persistent")
 PATH_NAMES = ("frontier", "direction_opt", "union_find")
 
 
@@ -129,7 +129,7 @@ class Stats(C.Structure):
         ("pr_residual", C.c_double),
         ("host_cores", C.c_int),
         ("path", C.c_int),
-        ("kern", KernelStats * 6),
+        ("kern", KernelStats * 7),
         ("reused_edges", C.c_uint64),
     ]
 

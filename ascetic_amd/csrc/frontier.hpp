@@ -116,8 +116,13 @@ struct Pushes {
     __shared__ uint32_t s_stage[3][kWavesPerBlock][kStageCap];      \
     Pushes<ALGO> pq{{s_stage[0][wave_id()], 0}, {s_stage[1][wave_id()], 0}, {s_stage[2][wave_id()], 0}}
 
+// Expansion of the resident frontier (slot level % 4) by workgroup `bid` of
+// `nblk`: 256 frontier vertices at a time, their edges walked edge-balanced;
+// vertices with degree >= heavy_thr go to the heavy list instead.  Called by
+// k_expand_local and by persistent kernels (with their own block numbering).
 template <int ALGO, class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
+__device__ __forceinline__ void expand_local_body(const TravArgs<OffT, EdgeT>& a, uint32_t level, uint32_t bid,
+                                                  uint32_t nblk) {
     __shared__ uint32_t s_pref[kBlock];
     __shared__ OffT s_beg[kBlock];
     __shared__ uint32_t s_sv[kBlock];
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
 
     Slot* cur = a.slots + (level & 3);
     Slot* nxt = a.slots + ((level + 1) & 3);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         Slot* z = a.slots + ((level + 2) & 3);
         z->n_res = 0;
         z->heavy = 0;
@@ -135,7 +140,7 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
     unsigned long long my_edges = 0, my_verts = 0;
 
-    for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    for (uint32_t base = bid * kBlock; base < n; base += nblk * kBlock) {
         const uint32_t i = base + tid;
         uint32_t deg = 0, sv = 0;
         OffT beg = 0;
@@ -209,7 +214,15 @@ __global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a
 }
 
 template <int ALGO, class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
+__global__ __launch_bounds__(kBlock) void k_expand_local(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    expand_local_body<ALGO>(a, level, blockIdx.x, gridDim.x);
+}
+
+// Grid-cooperative expansion of the heavy list of slot level % 4 by
+// workgroup `bid` of `nblk`.
+template <int ALGO, class OffT, class EdgeT>
+__device__ __forceinline__ void expand_heavy_body(const TravArgs<OffT, EdgeT>& a, uint32_t level, uint32_t bid,
+                                                  uint32_t nblk) {
     constexpr int kWin = 8;  // a tile spans <= kTile / kMinHeavy + 1 heavy entries
     __shared__ unsigned long long s_hb[kWin];
     __shared__ OffT s_hoff[kWin];
@@ -229,8 +242,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
     // contiguous runs of tiles per workgroup: only the first tile searches
     // the heavy list; later tiles start at the previous tile's last owner
     const uint64_t n_tiles = (total + kTile - 1) / kTile;
-    const uint64_t per = (n_tiles + gridDim.x - 1) / gridDim.x;
-    const uint64_t tb = min(n_tiles, (uint64_t)blockIdx.x * per), te = min(n_tiles, tb + per);
+    const uint64_t per = (n_tiles + nblk - 1) / nblk;
+    const uint64_t tb = min(n_tiles, (uint64_t)bid * per), te = min(n_tiles, tb + per);
     uint32_t k_last = 0;
     for (uint64_t t = tb; t < te; t++) {
         const uint64_t t0 = t * kTile;
@@ -297,6 +310,11 @@ __global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a
     }
     pq.flush(a, nxt);
     if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
+}
+
+template <int ALGO, class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_expand_heavy(TravArgs<OffT, EdgeT> a, uint32_t level) {
+    expand_heavy_body<ALGO>(a, level, blockIdx.x, gridDim.x);
 }
 
 }  // namespace asc

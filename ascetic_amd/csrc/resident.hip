@@ -181,16 +181,19 @@ __device__ __forceinline__ void block_add(unsigned long long x, unsigned long lo
 // waits for one lane walking a long list edge by edge.
 constexpr int kBuLane = 8;
 
+// Body of one bottom-up level for workgroup `bid` of `nblk`.  With q_out set,
+// the vertices found are also appended to that queue (count in *q_count, one
+// atomic per wave), so a top-down level can follow without a conversion.
 template <class OffT, class EdgeT>
-__global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const EdgeT* edges, uint32_t* val,
-                                                          uint32_t* visited, const uint32_t* front, uint32_t* next,
-                                                          uint32_t V, uint32_t level, DoCounters* c,
-                                                          DevStats* stats) {
+__device__ __forceinline__ void bottom_up_body(const OffT* off, const EdgeT* edges, uint32_t* val, uint32_t* visited,
+                                               const uint32_t* front, uint32_t* next, uint32_t V, uint32_t level,
+                                               DoCounters* c, DevStats* stats, uint32_t* q_out, uint32_t* q_count,
+                                               uint32_t bid, uint32_t nblk) {
     __shared__ unsigned long long s_tmp[kWavesPerBlock];
     const uint32_t lane = lane_id();
-    const uint32_t nwaves = gridDim.x * kWavesPerBlock;
+    const uint32_t nwaves = nblk * kWavesPerBlock;
     unsigned long long n_found = 0, m_found = 0, examined = 0, scanned = 0;
-    for (uint64_t wv = (uint64_t)blockIdx.x * kWavesPerBlock + wave_id(); wv * kWave < V; wv += nwaves) {
+    for (uint64_t wv = (uint64_t)bid * kWavesPerBlock + wave_id(); wv * kWave < V; wv += nwaves) {
         const uint32_t base = (uint32_t)(wv * kWave);
         const uint32_t v = base + lane;
         bool found = false;
@@ -252,11 +255,26 @@ __global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const
                 if (word) visited[wi] |= word;
             }
         }
+        if (q_out && m) {
+            uint32_t at = 0;
+            if (lane == 0) at = atomicAdd(q_count, (uint32_t)__popcll(m));
+            at = __shfl(at, 0, kWave);
+            if (found) q_out[at + lanes_below(m)] = v;
+        }
     }
     block_add(n_found, s_tmp, &c->n_found);
     block_add(m_found, s_tmp, &c->m_found);
     block_add(examined, s_tmp, &stats->edges_bu);
     block_add(scanned, s_tmp, &stats->verts_bu);
+}
+
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_bfs_bottom_up(const OffT* off, const EdgeT* edges, uint32_t* val,
+                                                          uint32_t* visited, const uint32_t* front, uint32_t* next,
+                                                          uint32_t V, uint32_t level, DoCounters* c,
+                                                          DevStats* stats) {
+    bottom_up_body(off, edges, val, visited, front, next, V, level, c, stats, (uint32_t*)nullptr,
+                   (uint32_t*)nullptr, blockIdx.x, gridDim.x);
 }
 
 // Frontier queue (count in slot->n_res) -> bitmap (cleared beforehand).
@@ -334,6 +352,118 @@ __global__ void k_do_publish(DoCounters* c, const Slot* next_slot, DoCounters* h
 }
 
 // Level-0 frontier: the source alone, value 1 (level + 1).
+// ---- persistent direction-optimizing BFS -----------------------------------
+// The whole traversal in one cooperative launch: every level runs between
+// grid barriers, and every workgroup takes the same direction decision from
+// counters that are final at the barrier, so the host only launches the
+// kernel and waits for it once.  A level is either
+//   top-down: expand_local_body over the frontier queue (heavy vertices to
+//     the heavy list), a barrier, and, when the list is not empty,
+//     expand_heavy_body and a second barrier;
+//   bottom-up: bottom_up_body over the frontier bitmap, which also appends
+//     the vertices it finds to the next queue, and a barrier.
+// Switching to bottom-up converts the current queue into its bitmap (one
+// more barrier); switching back needs nothing, the queue is already there.
+// Three bitmaps rotate: level L reads bm[L % 3], writes bm[(L + 1) % 3] and
+// clears bm[(L + 2) % 3].  The decision uses vertex counts: bottom-up once
+// the frontier holds more than V / kAlphaV vertices, top-down again when it
+// is below V / kBeta and shrinking.  Barrier waits are bounded: a workgroup
+// that waits past kBarSpin polls sets the error word, every workgroup then
+// leaves at its next barrier, and the host reports a device error.
+constexpr uint32_t kAlphaV = 32;
+constexpr uint32_t kBarSpin = 1u << 22;
+constexpr int kPctlWords = 8;   // [0] barrier count, [1] barrier generation, [2] error, [3] levels run
+
+__device__ __forceinline__ bool grid_barrier(uint32_t* ctl, uint32_t nblk) {
+    __shared__ uint32_t s_ok;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        __threadfence();
+        const uint32_t gen = __hip_atomic_load(ctl + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        if (__hip_atomic_fetch_add(ctl, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1 == nblk) {
+            __hip_atomic_store(ctl, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(ctl + 1, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint32_t spins = 0;
+            while (__hip_atomic_load(ctl + 1, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
+                if (__hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) || ++spins > kBarSpin) {
+                    __hip_atomic_store(ctl + 2, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+        __threadfence();
+        if (__hip_atomic_load(ctl + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) ok = 0;
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+__device__ __forceinline__ uint32_t ld_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <class OffT, class EdgeT>
+__global__ __launch_bounds__(kBlock) void k_bfs_persistent(TravArgs<OffT, EdgeT> a, uint32_t* q0, uint32_t* q1,
+                                                           uint32_t* bm0, uint32_t* bm1, uint32_t* bm2,
+                                                           DoCounters* cnt, uint32_t* ctl, uint32_t V,
+                                                           uint32_t max_levels) {
+    const uint32_t bid = blockIdx.x, nblk = gridDim.x;
+    uint32_t* bm[3] = {bm0, bm1, bm2};
+    uint32_t* q[2] = {q0, q1};
+    const uint32_t words = (V + 31) / 32;
+    bool bu = false;
+    uint32_t prev_nf = 0, level = 0;
+    for (; level < max_levels; level++) {
+        Slot* cur = a.slots + (level & 3);
+        const uint32_t nf = ld_agent(&cur->n_res);
+        if (nf == 0) break;
+        const bool to_bu = !bu && level > 0 && nf > V / kAlphaV;
+        if (bu && nf < V / (uint32_t)kBeta && nf < prev_nf) bu = false;
+        prev_nf = nf;
+        // the bitmap level + 1 writes is cleared now (a bottom-up level
+        // writes whole words, a top-down one sets single bits of it)
+        uint32_t* bz = bm[(level + 2) % 3];
+        for (uint32_t i = bid * kBlock + threadIdx.x; i < words; i += nblk * kBlock) bz[i] = 0;
+        if (to_bu) {
+            // frontier queue -> bitmap bm[level % 3] (cleared at level - 2)
+            for (uint32_t i = bid * kBlock + threadIdx.x; i < nf; i += nblk * kBlock) {
+                const uint32_t v = q[level & 1][i];
+                atomicOr(bm[level % 3] + (v >> 5), 1u << (v & 31));
+            }
+            bu = true;
+            if (!grid_barrier(ctl, nblk)) return;
+        }
+        if (bu) {
+            if (bid == 0 && threadIdx.x == 0) {
+                Slot* z = a.slots + ((level + 2) & 3);
+                z->n_res = 0;
+                z->heavy = 0;
+            }
+            bottom_up_body(a.off, a.edges, a.val, a.visited, bm[level % 3], bm[(level + 1) % 3], V, level, cnt,
+                           a.stats, q[(level + 1) & 1], &a.slots[(level + 1) & 3].n_res, bid, nblk);
+            if (!grid_barrier(ctl, nblk)) return;
+        } else {
+            TravArgs<OffT, EdgeT> b = a;
+            b.q_in = q[level & 1];
+            b.q_out = q[(level + 1) & 1];
+            expand_local_body<ALG_BFS>(b, level, bid, nblk);
+            if (!grid_barrier(ctl, nblk)) return;
+            const unsigned long long h = __hip_atomic_load(&cur->heavy, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (h >> kHeavyShift) {
+                expand_heavy_body<ALG_BFS>(b, level, bid, nblk);
+                if (!grid_barrier(ctl, nblk)) return;
+            }
+            // a top-down level's output becomes a bitmap only if bottom-up follows
+        }
+    }
+    if (bid == 0 && threadIdx.x == 0) ctl[3] = level;
+}
+
 __global__ void k_do_source(uint32_t* val, uint32_t* visited, uint32_t* q, Slot* slot, uint32_t src) {
     val[src] = 1;
     visited[src >> 5] |= 1u << (src & 31);
@@ -348,6 +478,8 @@ static int ensure_fast_state(asc_graph* g) {
     if (!t->bm[0]) {
         ASC_HIP(hipMalloc(&t->bm[0], 4 * words));
         ASC_HIP(hipMalloc(&t->bm[1], 4 * words));
+        ASC_HIP(hipMalloc(&t->bm[2], 4 * words));
+        ASC_HIP(hipMalloc(&t->pctl, 4 * kPctlWords));
     }
     if (!t->dcnt) {
         ASC_HIP(hipMalloc(&t->dcnt, sizeof(DoCounters)));
@@ -478,8 +610,98 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
     return finish_traversal(g, ALG_BFS, PATH_DIRECTION_OPT, out, st, rc, t1 - t0, level);
 }
 
+// The persistent form: initial state, one cooperative launch, one wait.
+// Returns ASC_ERR_UNSUPPORTED (nothing run) when the device refuses a
+// cooperative launch of the kernel, so the caller steps levels from the host.
+template <class OffT, class EdgeT>
+static int run_bfs_persistent(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) {
+    asc_ctx* ctx = g->ctx;
+    ASC_HIP(hipSetDevice(ctx->device));
+    ASC_TRY(ensure_fast_state(g));
+    TravState* t = g->trav;
+    const uint64_t V = g->h.V;
+    const uint64_t words = (V + 31) / 32;
+    hipStream_t s = ctx->s_main;
+    const bool prof = ctx->cfg.profile != 0;
+    const int fill_grid = ctx->num_cus * 4;
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_bfs_persistent<OffT, EdgeT>, kBlock, 0) !=
+            hipSuccess ||
+        per_cu < 1)
+        return ASC_ERR_UNSUPPORTED;
+    const int grid = ctx->num_cus * std::min(per_cu, g->blocks_per_cu > 0 ? g->blocks_per_cu : kBlocksPerCU);
+
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->slots, sizeof(Slot) * 4 / 4, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->stats, sizeof(DevStats) / 4, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, (uint32_t*)t->dcnt, sizeof(DoCounters) / 4, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(1), dim3(kBlock), 0, s, t->pctl, (uint64_t)kPctlWords, 0u);
+    hipLaunchKernelGGL(k_do_fill, dim3(fill_grid), dim3(kBlock), 0, s, t->val, V, (uint32_t)(V + 1));
+    hipLaunchKernelGGL(k_do_fill, dim3(fill_grid), dim3(kBlock), 0, s, t->visited, words, 0u);
+    for (int i = 0; i < 3; i++)
+        hipLaunchKernelGGL(k_do_fill, dim3(fill_grid), dim3(kBlock), 0, s, t->bm[i], words, 0u);
+    hipLaunchKernelGGL(k_do_source, dim3(1), dim3(1), 0, s, t->val, t->visited, t->q[0], t->slots, src);
+    ASC_HIP(hipGetLastError());
+    ASC_HIP(hipStreamSynchronize(s));
+
+    TravArgs<OffT, EdgeT> a{};
+    a.off = (const OffT*)g->d_off;
+    a.edges = (const EdgeT*)g->d_edges;
+    a.val = t->val;
+    a.visited = t->visited;
+    a.slots = t->slots;
+    a.heavy_u = t->heavy_u;
+    a.heavy_sv = t->heavy_sv;
+    a.heavy_base = t->heavy_base;
+    a.stats = t->stats;
+    a.vs = (uint32_t)V;
+    a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
+    a.theta = 0xFFFFFFFFu;
+    uint32_t* q0 = t->q[0];
+    uint32_t* q1 = t->q[1];
+    uint32_t* b0 = t->bm[0];
+    uint32_t* b1 = t->bm[1];
+    uint32_t* b2 = t->bm[2];
+    DoCounters* cnt = t->dcnt;
+    uint32_t* ctl = t->pctl;
+    uint32_t V32 = (uint32_t)V;
+    uint32_t max_levels = (uint32_t)std::min<uint64_t>(V + 1, 0xFFFFFFFFull);
+    void* args[] = {&a, &q0, &q1, &b0, &b1, &b2, &cnt, &ctl, &V32, &max_levels};
+
+    RunCounters rc{};
+    t->timed.clear();
+    t->ev_next = 0;
+    hipEvent_t e0 = next_event(t), e1 = next_event(t);
+    if (!e0 || !e1) return ASC_ERR_DEVICE;
+    const double t0 = now_ms();
+    ASC_HIP(hipEventRecord(e0, s));
+    if (hipLaunchCooperativeKernel((const void*)k_bfs_persistent<OffT, EdgeT>, dim3(grid), dim3(kBlock), args, 0,
+                                   s) != hipSuccess) {
+        (void)hipGetLastError();
+        return ASC_ERR_UNSUPPORTED;
+    }
+    ASC_HIP(hipEventRecord(e1, s));
+    ASC_HIP(hipStreamSynchronize(s));
+    const double t1 = now_ms();
+    t->timed.push_back({prof ? (int)KERN_PERSIST : -1, e0, e1});
+    uint32_t h_ctl[kPctlWords];
+    ASC_HIP(hipMemcpy(h_ctl, ctl, sizeof(h_ctl), hipMemcpyDeviceToHost));
+    if (h_ctl[2]) {
+        set_last_error("persistent BFS: a grid barrier wait exceeded its bound");
+        return ASC_ERR_DEVICE;
+    }
+    return finish_traversal(g, ALG_BFS, PATH_DIRECTION_OPT, out, st, rc, t1 - t0, h_ctl[3]);
+}
+
 int bfs_direction_opt(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) {
     const bool w = g->h.kind == ASC_KIND_WCSR, o4 = g->off_bytes == 4;
+    if (!g->ctx->cfg.bfs_host_stepped) {
+        int r;
+        if (w) r = o4 ? run_bfs_persistent<uint32_t, WEdge>(g, src, out, st)
+                      : run_bfs_persistent<uint64_t, WEdge>(g, src, out, st);
+        else r = o4 ? run_bfs_persistent<uint32_t, uint32_t>(g, src, out, st)
+                    : run_bfs_persistent<uint64_t, uint32_t>(g, src, out, st);
+        if (r != ASC_ERR_UNSUPPORTED) return r;
+    }
     if (w) return o4 ? run_bfs_do<uint32_t, WEdge>(g, src, out, st) : run_bfs_do<uint64_t, WEdge>(g, src, out, st);
     return o4 ? run_bfs_do<uint32_t, uint32_t>(g, src, out, st) : run_bfs_do<uint64_t, uint32_t>(g, src, out, st);
 }

@@ -460,7 +460,7 @@ void trav_free(TravState* t) {
     for (int i = 0; i < kPartBufs; i++) f(t->part[i]);
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
     f(t->od_list); f(t->od_prefix); f(t->scan_tmp); f(t->slots); f(t->piles); f(t->stats);
-    f(t->bm[0]); f(t->bm[1]); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
+    f(t->bm[0]); f(t->bm[1]); f(t->bm[2]); f(t->pctl); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
     f(t->d_cursor);
     for (int i = 0; i < kPartBufs; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
@@ -1221,7 +1221,9 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
             kern[x.kind].launches++;
         }
     }
-    if (prof) static_ms = kern[KERN_LOCAL].ms + kern[KERN_HEAVY].ms + kern[KERN_BU].ms + kern[KERN_UF].ms;
+    if (prof)
+        static_ms = kern[KERN_LOCAL].ms + kern[KERN_HEAVY].ms + kern[KERN_BU].ms + kern[KERN_UF].ms +
+                    kern[KERN_PERSIST].ms;
     ASC_HIP(hipMemcpy(t->h_stats.p, t->stats, sizeof(DevStats), hipMemcpyDeviceToHost));
     const DevStats& ds = t->h_stats[0];
     kern[KERN_LOCAL].edges = ds.edges_local;

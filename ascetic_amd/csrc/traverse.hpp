@@ -48,7 +48,10 @@ struct DevStats {
     unsigned long long edges_uf;      // union-find link passes: edge records read
 };
 
-enum { KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3, KERN_BU = 4, KERN_UF = 5, KERN_COUNT = 6 };
+enum {
+    KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3, KERN_BU = 4, KERN_UF = 5, KERN_PERSIST = 6,
+    KERN_COUNT = 7
+};
 
 // asc_stats.path
 enum { PATH_FRONTIER = 0, PATH_DIRECTION_OPT = 1, PATH_UNION_FIND = 2 };
@@ -163,8 +166,10 @@ struct TravState {
     // rates the dense-chunk choice balances (bytes/s): pinned host-to-device
     // copies (measured once) and the host gather (running estimate)
     double link_bps = 0, gather_bps = 30e9;
-    // resident fast paths: frontier bitmaps, counters, sampled labels
-    uint32_t* bm[2] = {nullptr, nullptr};
+    // resident fast paths: frontier bitmaps (the persistent BFS keeps three in
+    // a ring), counters, sampled labels, the persistent kernel's control words
+    uint32_t* bm[3] = {nullptr, nullptr, nullptr};
+    uint32_t* pctl = nullptr;
     DoCounters* dcnt = nullptr;
     HostBuf<DoCounters> h_dcnt;     // written by the device through d_hdcnt (mapped)
     DoCounters* d_hdcnt = nullptr;

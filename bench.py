@@ -253,6 +253,12 @@ def kernel_bytes(st, name, s_e, s_o, V=0):
     read the id and two parent probes, per vertex and link pass an offset
     pair and its parent."""
     k = st["kernels"][name]
+    if name == "bfs_persistent":   # the whole BFS in one launch: its expansions' bytes, bitmaps once per level
+        levels = max(st.get("iterations", 1), 1)
+        kk = st["kernels"]
+        return (kk["expand_local"]["edges"] * (s_e + 4) + kk["expand_local"]["vertices"] * (2 * s_o + 12) +
+                kk["expand_heavy"]["edges"] * (s_e + 4) + kk["bottom_up"]["edges"] * (s_e + 4) +
+                kk["bottom_up"]["vertices"] * (2 * s_o + 4) + levels * V // 4)
     if name == "pr_pull":
         return k["edges"] * 8 + k["vertices"] * (2 * s_o + 16)
     if name == "bottom_up":

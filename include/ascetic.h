@@ -160,7 +160,11 @@ typedef struct {
                                     frontier after every streamed part or
                                     chunk, while the next one is in flight;
                                     0 = after the whole round                */
-    int pad_cfg;
+    int bfs_host_stepped;        /* direction-optimizing BFS: 0 (default) =
+                                    the whole traversal in one persistent
+                                    cooperative kernel that decides each
+                                    level's direction and the end itself;
+                                    1 = one host round trip per level        */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -275,8 +279,9 @@ typedef struct {
      * and vertex counts always:
      * [0] resident frontier expansion, [1] grid-cooperative heavy vertices,
      * [2] on-demand parts, [3] PageRank pull, [4] BFS bottom-up steps,
-     * [5] union-find link passes. */
-    asc_kernel_stats kern[6];
+     * [5] union-find link passes, [6] the persistent BFS kernel (whole
+     * traversal; its edges and vertices are counted under [0], [1], [4]). */
+    asc_kernel_stats kern[7];
     uint64_t reused_edges;       /* on-demand edges read again from a chunk an
                                     earlier round of the run left in a part
                                     buffer (no transfer; not in streamed_edges) */

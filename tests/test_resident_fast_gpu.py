@@ -89,6 +89,28 @@ def test_direction_opt_bfs_exact(gpu_ctx, plain_ctx, name, off, dst, off32):
 
 
 @pytest.mark.parametrize("name,off,dst", SYM, ids=[s[0] for s in SYM])
+def test_persistent_bfs_matches_host_stepped(name, off, dst):
+    """The direction-optimizing BFS run as one persistent cooperative kernel
+    (the default) and stepped level by level from the host give the oracle's
+    levels; the persistent one is a single launch."""
+    say("persistent-bfs", name)
+    csr = O.CSR(off, dst)
+    deg = np.diff(off)
+    srcs = sorted({0, int(np.argmax(deg)), len(off) - 2})
+    with A.Context(profile=True) as c1, A.Context(profile=True, bfs_host_stepped=True) as c2, \
+            c1.from_arrays(off, dst) as g1, c2.from_arrays(off, dst) as g2:
+        for s in srcs:
+            lv1, st1 = g1.bfs(s)
+            lv2, st2 = g2.bfs(s)
+            ref = O.bfs_queue(csr, s)
+            assert np.array_equal(lv1, ref) and np.array_equal(lv2, ref), (name, s)
+            assert st1["kernels"]["bfs_persistent"]["launches"] == 1
+            assert st2["kernels"]["bfs_persistent"]["launches"] == 0
+            assert st1["iterations"] == st2["iterations"]
+            assert st1["reached"] == st2["reached"] == int((ref != len(off)).sum())
+
+
+@pytest.mark.parametrize("name,off,dst", SYM, ids=[s[0] for s in SYM])
 @pytest.mark.parametrize("off32", [False, True])
 def test_union_find_cc_exact(gpu_ctx, plain_ctx, name, off, dst, off32):
     say("uf-cc", name, "off32", off32)
