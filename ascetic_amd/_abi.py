@@ -284,6 +284,7 @@ SIGNATURES = {
     "asc_graph_replan": (C.c_int, [P, C.c_uint64, C.c_double]),
     "asc_bfs": (C.c_int, [P, C.c_uint32, u32p, C.POINTER(Stats)]),
     "asc_sssp": (C.c_int, [P, C.c_uint32, u32p, C.POINTER(Stats)]),
+    "asc_bfs_noalloc": (C.c_int, [P, C.c_uint32, C.c_void_p, C.c_void_p]),
     "asc_cc": (C.c_int, [P, u32p, C.POINTER(Stats)]),
     "asc_pr_opts_init": (None, [C.POINTER(PrOpts)]),
     "asc_pr": (C.c_int, [P, C.POINTER(PrOpts), C.POINTER(C.c_float), C.POINTER(C.c_double),

@@ -282,6 +282,11 @@ int asc_ctx_create(const asc_config* cfg, asc_ctx** out) {
     }
     int nt = c.gather_threads > 0 ? c.gather_threads : std::min(16, hardware_threads());
     ctx->pool = new ThreadPool(nt);
+    const int rn = noalloc_init(c.device);   // once per device, with the device current
+    if (rn != ASC_OK) {
+        asc_ctx_destroy(ctx);
+        return rn;
+    }
     *out = ctx;
     return ASC_OK;
 }

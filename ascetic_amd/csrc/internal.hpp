@@ -250,6 +250,7 @@ struct asc_graph {
 namespace asc {
 int graph_upload(asc_graph* g);       // device copies after a (re)plan
 int graph_ready(const asc_graph* g);  // ASC_OK, or ASC_ERR_STATE for an unusable graph
+int noalloc_init(int device);          // device address of the no-allocation scratch (device current)
 void graph_free_device(asc_graph* g);
 void trav_free(TravState* t);
 void pr_free(PrState* p);

@@ -292,6 +292,20 @@ int asc_bfs(asc_graph* g, uint32_t source, uint32_t* level_out, asc_stats* st);
 int asc_sssp(asc_graph* g, uint32_t source, uint32_t* dist_out, asc_stats* st);
 int asc_cc(asc_graph* g, uint32_t* label_out, asc_stats* st);
 
+/* BFS that allocates nothing.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory at call time (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same levels as asc_bfs (value convention of the context), written into
+ * d_level_out, a DEVICE buffer of V u32 values owned by the caller; the
+ * frontier queues, visited bitmap and counters live in one statically sized
+ * device array per GPU (about 17 MB, set up by asc_ctx_create), so the call
+ * makes no hipMalloc / hipFree.  Limits: the graph must be fully resident
+ * (ASC_ERR_UNSUPPORTED otherwise) with V <= 2^21 (ASC_ERR_RANGE above).
+ * `stream`: a hipStream_t to run on, or NULL for the context's stream; the
+ * call returns after its work on that stream ended.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int asc_bfs_noalloc(asc_graph* g, uint32_t source, uint32_t* d_level_out, void* stream);
+
 typedef enum {
     ASC_PR_NORMALISED = 0,       /* r = (1-d)/V + d*(sum r/outdeg + dangling/V),
                                     fixed iterations, FP32 ranks, FP64 residual */
