@@ -255,7 +255,7 @@ class Tuning(C.Structure):
         ("pr_group", C.c_int),
         ("sssp_delta", C.c_int),
         ("serve_lanes", C.c_int),
-        ("pad", C.c_int),
+        ("pr_tile_src_bits", C.c_int),
         ("score_ms", C.c_double),
         ("untuned_ms", C.c_double),
         ("fingerprint", C.c_uint64),

@@ -98,6 +98,7 @@ int asc_tuning_apply(asc_graph* g, const asc_tuning* t) {
     if (t->pr_group > 0) g->pr_group = t->pr_group;
     if (t->sssp_delta != 0) g->sssp_delta = t->sssp_delta;
     if (t->serve_lanes > 0) g->serve_lanes = std::min(t->serve_lanes, 64);
+    if (t->pr_tile_src_bits > 0) g->pr_tile_src_bits = t->pr_tile_src_bits;
     return ASC_OK;
 }
 
@@ -145,6 +146,10 @@ int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out) {
             const int mw = std::max(1, (int)std::lround(g->mean_weight > 0 ? g->mean_weight : 32.0));
             sweep(&g->sssp_delta, {-1, std::max(1, mw / 2), mw, 4 * mw});
         }
+    } else if (g->static_vertices == g->h.V && g->h.V >= (uint64_t(1) << 20)) {
+        // the tiled pull (the default here): source segment size
+        if (g->pr_tile_src_bits <= 0) g->pr_tile_src_bits = 15;
+        sweep(&g->pr_tile_src_bits, {13, 14, 16, 17, 18});
     } else {
         sweep(&g->pr_group, {4, 8, 16, 32, 64});
     }
@@ -168,6 +173,7 @@ int asc_autotune(asc_graph* g, double budget_ms, asc_tuning* out) {
         out->pr_group = g->pr_group;
         out->sssp_delta = g->sssp_delta;
         out->serve_lanes = g->serve_lanes;
+        out->pr_tile_src_bits = g->pr_tile_src_bits;
         out->score_ms = best;
         out->untuned_ms = untuned;
         out->fingerprint = fp;
@@ -193,9 +199,10 @@ int asc_tuning_save(const char* path, const asc_graph* g, const asc_tuning* t) {
     char buf[512];
     snprintf(buf, sizeof(buf),
              "fingerprint %016llx V %llu E %llu heavy_threshold %d blocks_per_cu %d pr_group %d sssp_delta %d "
-             "serve_lanes %d score_ms %.3f untuned_ms %.3f",
+             "serve_lanes %d pr_tile_src_bits %d score_ms %.3f untuned_ms %.3f",
              (unsigned long long)fp, (unsigned long long)g->h.V, (unsigned long long)g->h.E, t->heavy_threshold,
-             t->blocks_per_cu, t->pr_group, t->sssp_delta, t->serve_lanes, t->score_ms, t->untuned_ms);
+             t->blocks_per_cu, t->pr_group, t->sssp_delta, t->serve_lanes, t->pr_tile_src_bits, t->score_ms,
+             t->untuned_ms);
     outf << buf << "\n";
     return outf.good() ? ASC_OK : ASC_ERR_IO;
 }
@@ -221,6 +228,7 @@ int asc_tuning_load(const char* path, asc_graph* g, asc_tuning* t) {
             else if (k == "pr_group") x.pr_group = std::stoi(v);
             else if (k == "sssp_delta") x.sssp_delta = std::stoi(v);
             else if (k == "serve_lanes") x.serve_lanes = std::stoi(v);
+            else if (k == "pr_tile_src_bits") x.pr_tile_src_bits = std::stoi(v);
             else if (k == "score_ms") x.score_ms = std::stod(v);
             else if (k == "untuned_ms") x.untuned_ms = std::stod(v);
         }

@@ -238,6 +238,8 @@ struct asc_graph {
     int sssp_delta = 0;
     double mean_weight = 0;         // SSSP: mean edge weight (0 = not computed)
     int serve_lanes = 64;
+    int pr_tile_src_bits = 0;       // PageRank tile shape (0 = default)
+    int pr_tile_dst_bits = 0;
     int blocks_per_cu = 0;          // workgroups per CU of traversal launches (0 = 8)
     uint32_t max_degree = 0, max_degree_vertex = 0;
     // adjacency symmetric (every u->v has a v->u of equal multiplicity):

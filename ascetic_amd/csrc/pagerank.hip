@@ -840,10 +840,19 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         set_last_error("propagation-blocked PageRank needs every edge resident, E < 2^32 and room for its slots");
         return ASC_ERR_UNSUPPORTED;
     }
-    const uint32_t sb = o->tile_src_bits > 0 ? (uint32_t)o->tile_src_bits : 18;
-    const uint32_t db = o->tile_dst_bits > 0 ? (uint32_t)o->tile_dst_bits : (uint32_t)kTlMaxDstBits;
-    if (o->method == ASC_PR_TILED) ASC_TRY(pr_tiled_build<OffT>(g, p, sb, db));
-    const bool tiled = o->method == ASC_PR_TILED && p->tl_state == 1;
+    // tile shape: 2^15-vertex segments (128 KB contribution slices) and
+    // 2^14-vertex blocks measured fastest on Erdos-Renyi 2^26 / 2^31 edges
+    // (20 iterations: 499 ms against 641-728 ms for 2^16-2^20-vertex segments
+    // and 821 ms for the pull); the graph's tuned shape or the caller's wins
+    const uint32_t sb = o->tile_src_bits > 0 ? (uint32_t)o->tile_src_bits
+                        : g->pr_tile_src_bits > 0 ? (uint32_t)g->pr_tile_src_bits : 15;
+    const uint32_t db = o->tile_dst_bits > 0 ? (uint32_t)o->tile_dst_bits
+                        : g->pr_tile_dst_bits > 0 ? (uint32_t)g->pr_tile_dst_bits : (uint32_t)kTlMaxDstBits;
+    // AUTO: the tiled pull once the contributions outgrow an XCD's L2
+    const bool want_tiled = o->method == ASC_PR_TILED ||
+                            (o->method == ASC_PR_AUTO && g->static_vertices == V && V >= (uint64_t(1) << 20));
+    if (want_tiled) ASC_TRY(pr_tiled_build<OffT>(g, p, sb, db));
+    const bool tiled = want_tiled && p->tl_state == 1;
     if (o->method == ASC_PR_TILED && !tiled) {
         set_last_error("source-segmented PageRank needs every edge resident, V <= 2^31 and room for its tiles");
         return ASC_ERR_UNSUPPORTED;

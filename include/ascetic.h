@@ -328,7 +328,9 @@ typedef struct {
 
 /* Normalised PageRank schedule (asc_pr_opts.method). */
 typedef enum {
-    ASC_PR_AUTO = 0,     /* the pull (the faster one measured on MI355X)     */
+    ASC_PR_AUTO = 0,     /* ASC_PR_TILED when every edge is resident and
+                            V >= 2^20 (contributions larger than an XCD's
+                            L2), else the pull                              */
     ASC_PR_PULL = 1,     /* G-lane pull over the in-CSR                      */
     ASC_PR_BLOCKED = 2,  /* propagation-blocked (scatter to slots by
                             destination bin, LDS sums per bin); needs every
@@ -512,7 +514,8 @@ typedef struct {
     int pr_group;                /* lanes per vertex of the PageRank pull     */
     int sssp_delta;              /* SSSP band width (-1 = no far pile)        */
     int serve_lanes;             /* serving batch width                       */
-    int pad;
+    int pr_tile_src_bits;        /* PageRank tiled pull: log2 of the source
+                                    segment (ASC_PR_TILED)                   */
     double score_ms;             /* summed time of the sampled workload with
                                     the chosen values                         */
     double untuned_ms;           /* the same workload with the defaults       */
