@@ -204,7 +204,7 @@ class Context:
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
                  heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=1,
                  bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO, dense_threshold=0.0, stream_overlap=None,
-                 bfs_host_stepped=False):
+                 bfs_host_stepped=False, release_host_edges=False):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -220,6 +220,7 @@ class Context:
         if stream_overlap is not None:
             cfg.stream_overlap = int(stream_overlap)
         cfg.bfs_host_stepped = int(bool(bfs_host_stepped))
+        cfg.release_host_edges = int(bool(release_host_edges))
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")
         self._h = h

@@ -58,6 +58,8 @@ class Config(C.Structure):
         ("dense_threshold", C.c_double),
         ("stream_overlap", C.c_int),
         ("bfs_host_stepped", C.c_int),
+        ("release_host_edges", C.c_int),
+        ("pad_cfg", C.c_int),
     ]
 
 
@@ -89,6 +91,7 @@ class GraphInfo(C.Structure):
         ("max_degree_vertex", C.c_uint32),
         ("symmetric", C.c_int),
         ("pad", C.c_int),
+        ("host_bytes", C.c_uint64),
     ]
 
 

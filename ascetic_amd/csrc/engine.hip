@@ -174,6 +174,15 @@ int graph_upload(asc_graph* g) {
             std::memcpy(g->zc_tail.p + b, src + b, 4 * (en - b));
         });
     }
+    // the host {dst, w} records are redundant once the on-demand edges sit in
+    // their packed pinned form and the static prefix on the device: free them
+    // on request (what later calls read from them is derived first)
+    if (ctx->cfg.release_host_edges && g->packed && g->h.edges.p) {
+        compute_mean_weight(g);
+        g->fp = fingerprint(g);
+        g->h.edges.release();
+        g->host_edges_released = true;
+    }
     // symmetry decides the resident fast paths (direction-optimizing BFS,
     // union-find CC); checked on the device copy when every edge is on it
     g->symmetric = -1;
@@ -418,11 +427,18 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info) {
     info->max_degree = g->max_degree;
     info->max_degree_vertex = g->max_degree_vertex;
     info->symmetric = g->symmetric;
+    info->host_bytes = 8 * (g->h.V + 1) + (g->h.edges.p ? (uint64_t)g->h.E * g->h.edge_bytes() : 0) +
+                       4ull * g->h.outdeg.n + 4ull * g->ptail.n + g->ptail_w.n + 4ull * g->zc_tail.n;
     return ASC_OK;
 }
 
 int asc_graph_replan(asc_graph* g, uint64_t edge_budget_bytes, double advise_rate) {
     if (!g) return ASC_ERR_INVALID;
+    if (g->host_edges_released) {
+        set_last_error("the graph's host edge copy was released (asc_config.release_host_edges); it cannot be "
+                       "re-planned");
+        return ASC_ERR_UNSUPPORTED;
+    }
     if (g->servers.load() > 0) {
         set_last_error("a server still runs over this graph; destroy it before re-planning");
         return ASC_ERR_STATE;

@@ -134,6 +134,8 @@ void asc_config_init(asc_config* c) {
     c->dense_threshold = 0.0;
     c->stream_overlap = 1;
     c->bfs_host_stepped = 0;
+    c->release_host_edges = 0;
+    c->pad_cfg = 0;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

@@ -238,6 +238,8 @@ struct asc_graph {
     int sssp_delta = 0;
     double mean_weight = 0;         // SSSP: mean edge weight (0 = not computed)
     int serve_lanes = 64;
+    bool host_edges_released = false;   // h.edges freed after packing (asc_config.release_host_edges)
+    uint64_t fp = 0;                // graph fingerprint (cached before a release)
     int pr_tile_src_bits = 0;       // PageRank tile shape (0 = default)
     int pr_tile_dst_bits = 0;
     int blocks_per_cu = 0;          // workgroups per CU of traversal launches (0 = 8)
@@ -253,6 +255,8 @@ namespace asc {
 int graph_upload(asc_graph* g);       // device copies after a (re)plan
 int graph_ready(const asc_graph* g);  // ASC_OK, or ASC_ERR_STATE for an unusable graph
 int noalloc_init(int device);          // device address of the no-allocation scratch (device current)
+uint64_t fingerprint(const asc_graph* g);   // sampled hash of the graph (tuning tables)
+void compute_mean_weight(asc_graph* g);    // mean edge weight, once (SSSP band width)
 void graph_free_device(asc_graph* g);
 void trav_free(TravState* t);
 void pr_free(PrState* p);

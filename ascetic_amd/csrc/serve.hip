@@ -832,12 +832,15 @@ static void engine_free(Engine& e) {
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
-    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt); f(e.d_ovf); f(e.d_sp);
+    // nothing of the engine may still run, and no recorded step may outlive
+    // the buffers it names
+    if (e.stream) (void)hipStreamSynchronize(e.stream);
     for (hipGraphExec_t& x : e.graph) {
         if (x) (void)hipGraphExecDestroy(x);
         x = nullptr;
     }
+    f(e.bits[0]); f(e.bits[1]); f(e.seen); f(e.lanes_state); f(e.q[0]); f(e.q[1]); f(e.slots);
+    f(e.heavy_u); f(e.heavy_bits); f(e.heavy_base); f(e.s0); f(e.staging); f(e.d_cnt); f(e.d_ovf); f(e.d_sp);
     if (e.stream) (void)hipStreamDestroy(e.stream);
     e.stream = nullptr;
 }

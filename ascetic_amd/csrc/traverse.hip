@@ -596,19 +596,22 @@ int effective_heavy_threshold(const asc_graph* g) {
 
 // SSSP bucket width: configured, else a multiple of the mean edge weight
 // (computed once per graph); a negative setting disables the far pile.
+void compute_mean_weight(asc_graph* g) {
+    if (g->mean_weight > 0 || g->h.kind != ASC_KIND_WCSR || !g->h.edges.p) return;
+    const uint64_t E = g->h.E;
+    std::atomic<uint64_t> sum{0};
+    host_pool().parallel_for(E, [&](uint64_t b, uint64_t e) {
+        uint64_t s = 0;
+        for (uint64_t i = b; i < e; i++) s += g->h.edges[2 * i + 1];
+        sum += s;
+    });
+    g->mean_weight = E ? (double)sum.load() / (double)E : 1.0;
+}
+
 static uint32_t sssp_delta(asc_graph* g) {
     if (g->sssp_delta > 0) return (uint32_t)g->sssp_delta;
     if (g->sssp_delta < 0) return 0;
-    if (g->mean_weight <= 0) {
-        const uint64_t E = g->h.E;
-        std::atomic<uint64_t> sum{0};
-        host_pool().parallel_for(E, [&](uint64_t b, uint64_t e) {
-            uint64_t s = 0;
-            for (uint64_t i = b; i < e; i++) s += g->h.edges[2 * i + 1];
-            sum += s;
-        });
-        g->mean_weight = E ? (double)sum.load() / (double)E : 1.0;
-    }
+    compute_mean_weight(g);
     // streamed graphs take wide bands (each band re-streams the on-demand
     // vertices it improves, so fewer rounds win; streamed vertices are
     // expanded with their value at expansion time, which limits the

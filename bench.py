@@ -617,10 +617,17 @@ def c4_shape_section(A, _abi, O, args, cpu_threads):
                     f"[1, 255], seed {args.seed} (the scale-30 configuration's shape at the largest scale one "
                     f"benchmark run generates and holds)",
            "V": V, "E_directed": E, "source": src, "edge_budget_bytes": budget}
-    with A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, profile=True) as ctx:
-        g = ctx.adopt(h, offset_bytes=8)
+    # the library copies the arrays and, once the on-demand edges sit in their
+    # packed pinned form, frees its {dst, w} copy (release_host_edges): the
+    # host footprint is then offsets + 5-byte records, which is what a scale-30
+    # graph needs; the generator's arrays stay with this script for the oracle
+    with A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, profile=True,
+                   release_host_edges=True) as ctx:
+        g = ctx.from_arrays(off, ew)
         info = g.info()
-        out.update({"static_edges": info["static_edges"], "ondemand_edges": info["ondemand_edges"]})
+        out.update({"static_edges": info["static_edges"], "ondemand_edges": info["ondemand_edges"],
+                    "library_host_bytes": info["host_bytes"], "device_bytes": info["device_bytes"],
+                    "edge_record_bytes": 8 * E})
         h2d, _ = ctx.measure_link(256 << 20, 8)
         csr = O.CSR(off, ew[:, 0], ew[:, 1])
         S, PE = info["static_edges"], info["part_edges"]
@@ -657,6 +664,7 @@ def c4_shape_section(A, _abi, O, args, cpu_threads):
             out[algo] = sec
             log(f"c4-shape {algo}: {sec['gteps']} GTEPS, {sec['ms_per_run']} ms/run, exact {sec['exact']}")
         g.close()
+    h.free()
     return out
 
 

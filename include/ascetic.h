@@ -165,6 +165,13 @@ typedef struct {
                                     cooperative kernel that decides each
                                     level's direction and the end itself;
                                     1 = one host round trip per level        */
+    int release_host_edges;      /* 1: once the on-demand edges are kept in
+                                    their packed pinned form, free the graph's
+                                    host copy of the {dst, w} records (it is
+                                    then no longer needed: the device holds
+                                    the static prefix); the graph can then not
+                                    be re-planned.  0 (default): keep it     */
+    int pad_cfg;
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -225,6 +232,9 @@ typedef struct {
                                     multiplicity; 0: not; -1: not checked
                                     (checked on upload when fully resident)  */
     int pad;
+    uint64_t host_bytes;         /* host memory the graph holds: offsets, the
+                                    edge records unless released, pinned
+                                    on-demand copies, out-degrees            */
 } asc_graph_info;
 int asc_graph_get_info(const asc_graph* g, asc_graph_info* info);
 

@@ -111,3 +111,29 @@ def test_split_form_chosen_for_wide_ids(mode):
             assert st["streamed_edges"] > 0 and st["streamed_bytes"] == 5 * st["streamed_edges"]
             lv, _ = g.bfs(s)
             assert np.array_equal(lv, O.bfs_queue(csr, s)), s
+
+
+@pytest.mark.parametrize("pack", [1, 2])
+def test_release_host_edges_after_packing(rmat_ef32, pack):
+    """asc_config.release_host_edges frees the graph's host {dst, w} copy once
+    the on-demand edges sit in their packed pinned form: the host then holds
+    offsets and packed records only, results are unchanged, and re-planning
+    (which would need the records) is refused."""
+    off, ew, src, d_ref, b_ref, _ = rmat_ef32
+    E = int(off[-1])
+    say("release host edges, pack", pack)
+    kw = dict(edge_budget_bytes=8 * E // 3, part_bytes=5 * 8192, stream_pack=pack)
+    with A.Context(**kw) as c0, c0.from_arrays(off, ew) as g0:
+        kept = g0.info()["host_bytes"]
+    with A.Context(release_host_edges=True, **kw) as ctx, ctx.from_arrays(off, ew) as g:
+        info = g.info()
+        rec = 5 if pack == 2 else 4
+        assert info["host_bytes"] == 8 * (len(off)) + rec * info["ondemand_edges"]
+        assert info["host_bytes"] < kept - 8 * E // 2
+        d, _ = g.sssp(src)
+        assert np.array_equal(d, d_ref)
+        lv, _ = g.bfs(src)
+        assert np.array_equal(lv, b_ref)
+        with pytest.raises(A.AscError) as e:
+            g.replan(edge_budget_bytes=8 * E // 2)
+        assert e.value.code == _abi.ERR_UNSUPPORTED
