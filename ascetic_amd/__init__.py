@@ -100,7 +100,10 @@ class HostGraph:
 
 
 def generate(type=0, scale=10, num_vertices=0, num_samples=0, edge_factor=16, a=0.57, b=0.19, c=0.19,
-             seed=1, symmetrize=True, weighted=False, wmin=1, wmax=64, csc=False, threads=0):
+             seed=1, symmetrize=True, weighted=False, wmin=1, wmax=64, csc=False, threads=0, device=None):
+    """Synthetic graph in host memory.  device: a Context whose GPU builds
+    it (asc_generate_device, the same CSR bit for bit); None builds it on
+    the host threads."""
     lib = _abi.load()
     p = _abi.GenParams()
     lib.asc_gen_params_init(C.byref(p))
@@ -109,7 +112,10 @@ def generate(type=0, scale=10, num_vertices=0, num_samples=0, edge_factor=16, a=
     p.symmetrize, p.weighted, p.wmin, p.wmax = int(symmetrize), int(weighted), wmin, wmax
     p.csc, p.threads = int(csc), threads
     h = _abi.HostCsr()
-    check(lib.asc_generate(C.byref(p), C.byref(h)), "asc_generate")
+    if device is not None:
+        check(lib.asc_generate_device(device._h, C.byref(p), C.byref(h)), "asc_generate_device")
+    else:
+        check(lib.asc_generate(C.byref(p), C.byref(h)), "asc_generate")
     return HostGraph(h)
 
 
@@ -204,7 +210,7 @@ class Context:
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
                  heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=1,
                  bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO, dense_threshold=0.0, stream_overlap=None,
-                 bfs_host_stepped=False, release_host_edges=False):
+                 bfs_persistent=False, release_host_edges=False):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -219,7 +225,7 @@ class Context:
         cfg.dense_threshold = dense_threshold
         if stream_overlap is not None:
             cfg.stream_overlap = int(stream_overlap)
-        cfg.bfs_host_stepped = int(bool(bfs_host_stepped))
+        cfg.bfs_persistent = int(bool(bfs_persistent))
         cfg.release_host_edges = int(bool(release_host_edges))
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")

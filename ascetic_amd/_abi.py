@@ -57,7 +57,7 @@ class Config(C.Structure):
         ("cc_method", C.c_int),
         ("dense_threshold", C.c_double),
         ("stream_overlap", C.c_int),
-        ("bfs_host_stepped", C.c_int),
+        ("bfs_persistent", C.c_int),
         ("release_host_edges", C.c_int),
         ("pad_cfg", C.c_int),
     ]
@@ -297,6 +297,7 @@ SIGNATURES = {
     "asc_split_parts": (C.c_int, [u64p, C.c_uint64, C.c_uint64, u64p, C.c_uint64, u64p]),
     "asc_gen_params_init": (None, [C.POINTER(GenParams)]),
     "asc_generate": (C.c_int, [C.POINTER(GenParams), C.POINTER(HostCsr)]),
+    "asc_generate_device": (C.c_int, [C.c_void_p, C.POINTER(GenParams), C.POINTER(HostCsr)]),
     "asc_read_graph": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(HostCsr)]),
     "asc_edge_list_opts_init": (None, [C.POINTER(EdgeListOpts)]),
     "asc_read_edge_list": (C.c_int, [C.c_char_p, C.POINTER(EdgeListOpts), C.POINTER(HostCsr)]),

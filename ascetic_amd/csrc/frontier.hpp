@@ -94,6 +94,24 @@ __device__ __forceinline__ void relax_commit_batch(const TravArgs<OffT, EdgeT>& 
         if (kind[j] != TO_NONE && prev[j] < sv[j]) to[j] = kind[j];
 }
 
+// Summed degrees of the vertices a BFS commit claimed (to == TO_RES).
+template <class OffT, class EdgeT>
+__device__ __forceinline__ unsigned long long claimed_degrees(const TravArgs<OffT, EdgeT>& a,
+                                                              const int (&to)[kUnroll],
+                                                              const uint32_t (&v)[kUnroll]) {
+    unsigned long long m = 0;
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++)
+        if (to[j] == TO_RES) m += (unsigned long long)(a.off[v[j] + 1] - a.off[v[j]]);
+    return m;
+}
+
+// One atomic per wave that claimed anything.
+__device__ __forceinline__ void add_claimed(unsigned long long* dst, unsigned long long m) {
+    m = wave_sum(m);
+    if (lane_id() == 0 && m) atomicAdd(dst, m);
+}
+
 template <int ALGO>
 struct Pushes {
     WaveStage<kStageCap> res, pend, far;
@@ -138,7 +156,7 @@ __device__ __forceinline__ void expand_local_body(const TravArgs<OffT, EdgeT>& a
     }
     const uint32_t n = cur->n_res;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    unsigned long long my_edges = 0, my_verts = 0;
+    unsigned long long my_edges = 0, my_verts = 0, m_claimed = 0;
 
     for (uint32_t base = bid * kBlock; base < n; base += nblk * kBlock) {
         const uint32_t i = base + tid;
@@ -201,12 +219,14 @@ __device__ __forceinline__ void expand_local_body(const TravArgs<OffT, EdgeT>& a
             relax_commit_batch<ALGO>(a, vv, cc, pr, valid, level, to);
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) pq.push(a, nxt, to[j], vv[j]);
+            if (ALGO == ALG_BFS && a.m_next) m_claimed += claimed_degrees(a, to, vv);
         }
         my_edges += total;
         my_verts += cnt;
         __syncthreads();
     }
     pq.flush(a, nxt);
+    if (ALGO == ALG_BFS && a.m_next) add_claimed(a.m_next, m_claimed);
     if (tid == 0 && my_verts) {
         atomicAdd(&a.stats->edges_local, my_edges);
         atomicAdd(&a.stats->verts_local, my_verts);
@@ -237,7 +257,7 @@ __device__ __forceinline__ void expand_heavy_body(const TravArgs<OffT, EdgeT>& a
     const uint64_t total = h & kHeavyMask;
     if (nh == 0) return;
     const uint32_t tid = threadIdx.x, lane = lane_id(), w = wave_id();
-    unsigned long long my_edges = 0;
+    unsigned long long my_edges = 0, m_claimed = 0;
 
     // contiguous runs of tiles per workgroup: only the first tile searches
     // the heavy list; later tiles start at the previous tile's last owner
@@ -304,11 +324,13 @@ __device__ __forceinline__ void expand_heavy_body(const TravArgs<OffT, EdgeT>& a
             relax_commit_batch<ALGO>(a, vv, cc, pr, valid, level, to);
 #pragma unroll
             for (int j = 0; j < kUnroll; j++) pq.push(a, nxt, to[j], vv[j]);
+            if (ALGO == ALG_BFS && a.m_next) m_claimed += claimed_degrees(a, to, vv);
         }
         my_edges += tlen;
         __syncthreads();
     }
     pq.flush(a, nxt);
+    if (ALGO == ALG_BFS && a.m_next) add_claimed(a.m_next, m_claimed);
     if (tid == 0 && my_edges) atomicAdd(&a.stats->edges_heavy, my_edges);
 }
 

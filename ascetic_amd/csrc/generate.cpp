@@ -1,61 +1,48 @@
-// Deterministic synthetic graphs: R-MAT and Erdos-Renyi.
+// Deterministic synthetic graphs: R-MAT and Erdos-Renyi, built on the host.
 //
-// Every sample is a pure function of (seed, sample index): a SplitMix64
-// finaliser over a counter, so generation is reproducible regardless of the
-// thread count, and the sample list is regenerated instead of stored between
-// the degree-count and the scatter pass.  R-MAT has no noise and no vertex
-// permutation (vertex 0 is the largest hub).  Self-loops and duplicates are
-// removed and every adjacency list is sorted.  Weights are drawn per
-// unordered pair so w(u,v) = w(v,u).
+// The samples come from the shared sampler (gen_sampler.hpp), so generation
+// is reproducible regardless of the thread count and the sample list is
+// regenerated instead of stored between the degree-count and the scatter
+// pass.  Self-loops and duplicates are removed and every adjacency list is
+// sorted.  gen_device.hip builds the same CSR on the GPU.
 #include <algorithm>
 
+#include "gen_sampler.hpp"
 #include "internal.hpp"
 
 namespace asc {
 
-namespace {
-
-inline uint64_t mix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-inline uint64_t rnd(uint64_t key, uint64_t ctr) { return mix64(key + ctr * 0x9E3779B97F4A7C15ull); }
-
-struct Sampler {
-    int type;
-    int scale;
-    uint64_t V;
-    uint64_t key;
-    uint32_t ta, tab, tabc;  // cumulative quadrant thresholds on 32-bit draws
-
-    // One undirected sample.
-    inline void sample(uint64_t i, uint32_t& u, uint32_t& v) const {
-        if (type == 1) {
-            uint64_t r0 = rnd(key, 2 * i), r1 = rnd(key, 2 * i + 1);
-            u = (uint32_t)(((unsigned __int128)r0 * V) >> 64);
-            v = (uint32_t)(((unsigned __int128)r1 * V) >> 64);
-            return;
-        }
-        uint32_t uu = 0, vv = 0;
-        uint64_t word = 0;
-        for (int l = 0; l < scale; l++) {
-            if ((l & 1) == 0) word = rnd(key, (i << 5) + (uint64_t)(l >> 1));
-            uint32_t r = (uint32_t)(word >> ((l & 1) * 32));
-            uint32_t row = r >= tab, col = (r >= ta && r < tab) || r >= tabc;
-            uu = (uu << 1) | row;
-            vv = (vv << 1) | col;
-        }
-        u = uu;
-        v = vv;
+int make_sampler(const asc_gen_params* p, Sampler& s, uint64_t& M) {
+    if (!p) return ASC_ERR_INVALID;
+    s = Sampler{};
+    s.type = p->type;
+    s.key = mix64(p->seed ^ 0x5A17EC0DEull);
+    if (p->type == 0) {
+        if (p->scale < 1 || p->scale > 31) return ASC_ERR_RANGE;
+        s.scale = p->scale;
+        s.V = uint64_t(1) << p->scale;
+        M = p->num_samples ? p->num_samples : (uint64_t)p->edge_factor * s.V;
+        double a = p->a, b = p->b, c = p->c;
+        if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return ASC_ERR_INVALID;
+        auto thr = [](double x) -> uint32_t {
+            double t = x * 4294967296.0;
+            return t >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)t;
+        };
+        s.ta = thr(a);
+        s.tab = thr(a + b);
+        s.tabc = thr(a + b + c);
+    } else if (p->type == 1) {
+        if (p->num_vertices == 0 || p->num_vertices >= 0xFFFFFFFEull) return ASC_ERR_RANGE;
+        s.V = p->num_vertices;
+        M = p->num_samples;
+    } else {
+        return ASC_ERR_INVALID;
     }
-};
-
-inline uint32_t pair_weight(uint64_t key, uint32_t a, uint32_t b, uint32_t wmin, uint32_t span) {
-    uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    uint64_t h = mix64(key ^ (((uint64_t)lo << 32) | hi));
-    return wmin + (uint32_t)(((unsigned __int128)h * span) >> 64);
+    if (p->weighted && (p->wmin == 0 || p->wmax < p->wmin)) return ASC_ERR_INVALID;
+    return ASC_OK;
 }
+
+namespace {
 
 // Exclusive scan of cnt[0..n) into off[0..n], in parallel.
 void parallel_scan(ThreadPool& pool, const uint32_t* cnt, uint64_t* off, uint64_t n) {
@@ -138,33 +125,9 @@ int build_csr(ThreadPool& pool, uint64_t V, uint64_t M, Emit emit, HostBuf<uint6
 }  // namespace
 
 int generate_graph(const asc_gen_params* p, HostCSR& out) {
-    if (!p) return ASC_ERR_INVALID;
-    Sampler s{};
-    s.type = p->type;
-    s.key = mix64(p->seed ^ 0x5A17EC0DEull);
-    uint64_t M;
-    if (p->type == 0) {
-        if (p->scale < 1 || p->scale > 31) return ASC_ERR_RANGE;
-        s.scale = p->scale;
-        s.V = uint64_t(1) << p->scale;
-        M = p->num_samples ? p->num_samples : (uint64_t)p->edge_factor * s.V;
-        double a = p->a, b = p->b, c = p->c;
-        if (a < 0 || b < 0 || c < 0 || a + b + c > 1.0) return ASC_ERR_INVALID;
-        auto thr = [](double x) -> uint32_t {
-            double t = x * 4294967296.0;
-            return t >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)t;
-        };
-        s.ta = thr(a);
-        s.tab = thr(a + b);
-        s.tabc = thr(a + b + c);
-    } else if (p->type == 1) {
-        if (p->num_vertices == 0 || p->num_vertices >= 0xFFFFFFFEull) return ASC_ERR_RANGE;
-        s.V = p->num_vertices;
-        M = p->num_samples;
-    } else {
-        return ASC_ERR_INVALID;
-    }
-    if (p->weighted && (p->wmin == 0 || p->wmax < p->wmin)) return ASC_ERR_INVALID;
+    Sampler s;
+    uint64_t M = 0;
+    ASC_TRY(make_sampler(p, s, M));
     const uint64_t V = s.V;
     ThreadPool local(p->threads > 0 ? p->threads : 1);
     ThreadPool& pool = p->threads > 0 ? local : host_pool();
@@ -210,7 +173,7 @@ int generate_graph(const asc_gen_params* p, HostCSR& out) {
         out.kind = ASC_KIND_WCSR;
         ASC_TRY(out.edges.alloc(2 * E + 1));
         const uint32_t span = p->wmax - p->wmin + 1;
-        const uint64_t wkey = mix64(p->seed ^ 0x3E16A7ull);
+        const uint64_t wkey = weight_key(p->seed);
         pool.parallel_for(V, [&](uint64_t b, uint64_t e) {
             for (uint64_t u = b; u < e; u++)
                 for (uint64_t k = off[u]; k < off[u + 1]; k++) {

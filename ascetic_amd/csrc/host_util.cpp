@@ -133,7 +133,7 @@ void asc_config_init(asc_config* c) {
     c->cc_method = ASC_CC_AUTO;
     c->dense_threshold = 0.0;
     c->stream_overlap = 1;
-    c->bfs_host_stepped = 0;
+    c->bfs_persistent = 0;
     c->release_host_edges = 0;
     c->pad_cfg = 0;
 }

@@ -85,6 +85,9 @@ struct TravArgs {
     uint32_t* heavy_sv;
     unsigned long long* heavy_base;
     DevStats* stats;
+    // BFS, when set: the expansion adds the degrees of the vertices it
+    // claims here (the next frontier's edges, for the direction decision)
+    unsigned long long* m_next;
     uint32_t vs;                    // vertices [0, vs) are resident
     uint32_t heavy_thr;
     uint32_t theta;                 // SSSP near/far boundary (UINT_MAX: everything near)

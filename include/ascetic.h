@@ -160,11 +160,14 @@ typedef struct {
                                     frontier after every streamed part or
                                     chunk, while the next one is in flight;
                                     0 = after the whole round                */
-    int bfs_host_stepped;        /* direction-optimizing BFS: 0 (default) =
-                                    the whole traversal in one persistent
-                                    cooperative kernel that decides each
-                                    level's direction and the end itself;
-                                    1 = one host round trip per level        */
+    int bfs_persistent;          /* direction-optimizing BFS: 1 = the whole
+                                    traversal in one persistent cooperative
+                                    kernel that decides each level's direction
+                                    and the end itself (same levels); 0
+                                    (default) = one host round trip per level,
+                                    faster on MI355X because its kernels run
+                                    at full occupancy and a grid barrier costs
+                                    about as much as a round trip            */
     int release_host_edges;      /* 1: once the on-demand edges are kept in
                                     their packed pinned form, free the graph's
                                     host copy of the {dst, w} records (it is
@@ -417,6 +420,14 @@ typedef struct {
 
 /* Generates a graph: self-loops and duplicate edges removed, adjacency sorted. */
 int asc_generate(const asc_gen_params* p, asc_host_csr* out);
+/* The same graph as asc_generate, bit for bit, built on the context's GPU
+ * (raw degrees, per vertex range the raw pairs as sort keys, radix sort,
+ * unique, records) and returned in host memory owned by the caller (free
+ * with asc_host_csr_free).  The in-CSR (csc) only of symmetrized graphs
+ * (ASC_ERR_UNSUPPORTED otherwise).  No counterpart in the reference, which
+ * reads its graphs from files; it makes the large synthetic inputs of the
+ * tests and benchmarks in seconds. */
+int asc_generate_device(asc_ctx* ctx, const asc_gen_params* p, asc_host_csr* out);
 
 /* ---- edge-list conversion (converter/converter.cpp:22-420) -------------- */
 typedef struct {

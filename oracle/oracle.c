@@ -29,6 +29,11 @@ void orc_set_fault(int fault) { g_fault = fault; }
  * iterations (0 = run to the end); its values are then partial. */
 static int g_iter_limit = 0;
 void orc_set_iteration_limit(int n) { g_iter_limit = n > 0 ? n : 0; }
+/* Timing samples: once this many edges are relaxed (or pulled), the
+ * remaining vertices of the pass are skipped and the run ends (0 = off). */
+static uint64_t g_edge_limit = 0;
+void orc_set_edge_limit(uint64_t n) { g_edge_limit = n; }
+static inline int over_edge_limit(uint64_t done) { return g_edge_limit && done >= g_edge_limit; }
 void orc_set_num_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 enum { ALG_BFS = 0, ALG_SSSP = 1, ALG_CC = 2 };
@@ -141,7 +146,8 @@ static int traverse_ref(const orc_graph* g, int algo, uint32_t src, uint64_t S, 
     for (uint64_t v = 0; v < V; v++) count += active[v];
     local.node_sum = count;
 
-    while (count && !(g_iter_limit > 0 && local.iterations >= (uint32_t)g_iter_limit)) {
+    while (count && !(g_iter_limit > 0 && local.iterations >= (uint32_t)g_iter_limit) &&
+           !over_edge_limit(local.edges_relaxed)) {
         local.iterations++;
         /* setStaticAndOverloadLabelBool + scan + compaction (CalculateOpt.cu:63-91):
          * lists in increasing vertex order. */
@@ -166,6 +172,7 @@ static int traverse_ref(const orc_graph* g, int algo, uint32_t src, uint64_t S, 
         uint64_t relaxed = 0;
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 64) reduction(+ : relaxed)
         for (uint64_t i = 0; i < ns; i++) {
+            if (over_edge_limit(local.edges_relaxed + relaxed)) continue;
             uint32_t u = slist[i];
             uint32_t sval = __atomic_load_n(&val[u], __ATOMIC_RELAXED);
             uint64_t b = g->off[u], e = g->off[u + 1];
@@ -179,7 +186,7 @@ static int traverse_ref(const orc_graph* g, int algo, uint32_t src, uint64_t S, 
         }
         local.edges_relaxed += relaxed;
 
-        if (no) {
+        if (no && !over_edge_limit(local.edges_relaxed)) {
             /* fillEdgeArrByMultiThread (GraphMeta.cuh:469-514) */
             if (ototal > gather_cap) {
                 free(gather);
@@ -210,6 +217,7 @@ static int traverse_ref(const orc_graph* g, int algo, uint32_t src, uint64_t S, 
                 relaxed = 0;
 #pragma omp parallel for num_threads(nt) schedule(dynamic, 64) reduction(+ : relaxed)
                 for (uint64_t i = first; i < first + cnt; i++) {
+                    if (over_edge_limit(local.edges_relaxed + relaxed)) continue;
                     uint32_t u = olist[i];
                     uint32_t sval = __atomic_load_n(&val[u], __ATOMIC_RELAXED);
                     /* part-relative offset, as in gpu_kernels.cuh:358-359 */
@@ -311,8 +319,12 @@ int orc_pr_ref(const orc_graph* g, const uint32_t* outdeg, double damping, doubl
     return it;
 }
 
+static uint64_t g_last_pulled = 0;
+uint64_t orc_last_pulled(void) { return g_last_pulled; }
+
 int orc_pr_norm(const orc_graph* g, const uint32_t* outdeg, double damping, int iters, double* rank,
                 double* residual) {
+    g_last_pulled = 0;
     const uint64_t V = g->V;
     if (V == 0) return -1;
     double* nr = (double*)malloc(sizeof(double) * V);
@@ -329,15 +341,20 @@ int orc_pr_norm(const orc_graph* g, const uint32_t* outdeg, double damping, int 
         if (g_fault == 5) dangling = 0;
         const double base = (1.0 - damping) / (double)V + damping * dangling / (double)V;
         double res = 0;
-#pragma omp parallel for reduction(+ : res)
+        uint64_t pulled = 0;
+#pragma omp parallel for reduction(+ : res, pulled)
         for (uint64_t v = 0; v < V; v++) {
+            if (over_edge_limit(pulled)) continue;
             double s = 0;
             for (uint64_t k = g->off[v]; k < g->off[v + 1]; k++) s += contrib[edst(g, k)];
+            pulled += g->off[v + 1] - g->off[v];
             nr[v] = base + damping * s;
             res += fabs(nr[v] - rank[v]);
         }
         memcpy(rank, nr, sizeof(double) * V);
         if (residual) residual[it] = res;
+        g_last_pulled += pulled;
+        if (over_edge_limit(pulled)) break;
     }
     free(nr);
     free(contrib);

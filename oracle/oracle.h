@@ -115,6 +115,13 @@ int orc_split_reference(const uint64_t* prefix, const uint32_t* deg, uint32_t n,
  * when n < 50); returns the edges copied and the copy's wall time. */
 void orc_set_iteration_limit(int n);
 void orc_set_num_threads(int n);
+/* orc_set_edge_limit(n): the restated traversals and orc_pr_norm skip the
+ * rest of a pass, and stop, once n edges are relaxed or pulled (0 = no
+ * limit); with one thread this bounds a timing sample to about n edges of
+ * the full-size graph.  orc_last_pulled(): edges the last orc_pr_norm
+ * call pulled. */
+void orc_set_edge_limit(uint64_t n);
+uint64_t orc_last_pulled(void);
 long long orc_gather_ref(const orc_graph* g, const uint32_t* list, uint64_t n, int nthreads, uint32_t* out,
                          double* copy_ms);
 

@@ -67,14 +67,17 @@ def load():
             "orc_split_reference": [u64p, u32p, C.c_uint32, C.c_uint64, C.c_uint64, u32p, u32p, u64p, C.c_int],
             "orc_set_fault": [C.c_int],
             "orc_set_iteration_limit": [C.c_int],
+            "orc_set_edge_limit": [C.c_uint64],
+            "orc_last_pulled": [],
             "orc_set_num_threads": [C.c_int],
             "orc_gather_ref": [G, u32p, C.c_uint64, C.c_int, u32p, f64p],
         }
-        void = ("orc_set_fault", "orc_set_iteration_limit", "orc_set_num_threads")
+        void = ("orc_set_fault", "orc_set_iteration_limit", "orc_set_num_threads", "orc_set_edge_limit")
         for name, args in sig.items():
             f = getattr(lib, name)
             f.argtypes = args
-            f.restype = None if name in void else (C.c_longlong if name == "orc_gather_ref" else C.c_int)
+            f.restype = (None if name in void else C.c_longlong if name == "orc_gather_ref"
+                         else C.c_uint64 if name == "orc_last_pulled" else C.c_int)
         _lib = lib
     return _lib
 
@@ -234,6 +237,25 @@ class iteration_limit:
 
     def __exit__(self, *exc):
         load().orc_set_iteration_limit(0)
+
+
+class edge_limit:
+    """Context manager bounding the restated traversals and pr_norm to about
+    n relaxed or pulled edges (single-thread timing samples at full size;
+    values are partial)."""
+
+    def __init__(self, n):
+        self.n = n
+
+    def __enter__(self):
+        load().orc_set_edge_limit(self.n)
+
+    def __exit__(self, *exc):
+        load().orc_set_edge_limit(0)
+
+
+def last_pulled():
+    return int(load().orc_last_pulled())
 
 
 def set_num_threads(n):

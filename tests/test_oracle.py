@@ -8,6 +8,7 @@ brute-force path enumeration on tiny graphs, fixed points of PageRank, and
 planted faults that each of these checks must catch.
 """
 import itertools
+import os
 
 import networkx as nx
 import numpy as np
@@ -348,3 +349,29 @@ def test_iteration_limit_bounds_the_restatement():
     assert lv[:4].tolist() == [1, 2, 3, 4] and (lv[4:] == 21).all()
     lv, st = O.bfs_ref(csr, 0)
     assert lv.tolist() == list(range(1, 21))
+
+
+def test_edge_limit_bounds_the_restatement():
+    """On one thread an edge limit of n stops the restated traversal and the
+    FP64 power iteration after about n edges (the last vertex started is
+    finished), and a limit of 0 leaves the whole run."""
+    off, dst = G.random_graph(4096, 40000, seed=5)[:2]
+    csr = O.CSR(off, dst)
+    full, st_full = O.cc_ref(csr, nthreads=1)
+    with O.edge_limit(5000):
+        _, st = O.cc_ref(csr, nthreads=1)
+    maxdeg = int(np.diff(off).max())
+    assert 5000 <= st["edges_relaxed"] < 5000 + maxdeg
+    assert st["iterations"] == 1
+    again, st2 = O.cc_ref(csr, nthreads=1)
+    assert np.array_equal(again, full) and st2["edges_relaxed"] == st_full["edges_relaxed"]
+    outdeg = np.diff(off).astype(np.uint32)
+    O.set_num_threads(1)
+    try:
+        with O.edge_limit(7000):
+            O.pr_norm(csr, outdeg, 0.85, 5)
+        assert 7000 <= O.last_pulled() < 7000 + maxdeg
+        O.pr_norm(csr, outdeg, 0.85, 2)
+        assert O.last_pulled() == 2 * len(dst)
+    finally:
+        O.set_num_threads(os.cpu_count() or 1)

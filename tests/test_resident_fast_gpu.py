@@ -91,13 +91,14 @@ def test_direction_opt_bfs_exact(gpu_ctx, plain_ctx, name, off, dst, off32):
 @pytest.mark.parametrize("name,off,dst", SYM, ids=[s[0] for s in SYM])
 def test_persistent_bfs_matches_host_stepped(name, off, dst):
     """The direction-optimizing BFS run as one persistent cooperative kernel
-    (the default) and stepped level by level from the host give the oracle's
-    levels; the persistent one is a single launch."""
+    (asc_config.bfs_persistent) and stepped level by level from the host (the
+    default) give the oracle's levels in the same number of levels; the
+    persistent one is a single launch."""
     say("persistent-bfs", name)
     csr = O.CSR(off, dst)
     deg = np.diff(off)
     srcs = sorted({0, int(np.argmax(deg)), len(off) - 2})
-    with A.Context(profile=True) as c1, A.Context(profile=True, bfs_host_stepped=True) as c2, \
+    with A.Context(profile=True, bfs_persistent=True) as c1, A.Context(profile=True) as c2, \
             c1.from_arrays(off, dst) as g1, c2.from_arrays(off, dst) as g2:
         for s in srcs:
             lv1, st1 = g1.bfs(s)
