@@ -210,7 +210,7 @@ class Context:
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
                  heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=1,
                  bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO, dense_threshold=0.0, stream_overlap=None,
-                 bfs_persistent=False, release_host_edges=False):
+                 bfs_persistent=False, release_host_edges=False, part_buffers=0):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -227,6 +227,7 @@ class Context:
             cfg.stream_overlap = int(stream_overlap)
         cfg.bfs_persistent = int(bool(bfs_persistent))
         cfg.release_host_edges = int(bool(release_host_edges))
+        cfg.part_buffers = int(part_buffers)
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")
         self._h = h
@@ -269,6 +270,22 @@ class Context:
         check(_abi.load().asc_graph_adopt(self._h, C.byref(host_graph._h), offset_bytes, C.byref(h)),
               "asc_graph_adopt")
         host_graph._owned = False
+        return Graph(self, h)
+
+    def generate_graph(self, type=0, scale=10, num_vertices=0, num_samples=0, edge_factor=16, a=0.57, b=0.19,
+                       c=0.19, seed=1, symmetrize=True, wmin=1, wmax=64, offset_bytes=8):
+        """Weighted synthetic graph generated straight into a device graph
+        (asc_graph_generate): static prefix on the GPU, packed on-demand
+        edges in pinned host memory, no host {dst, w} records."""
+        lib = _abi.load()
+        p = _abi.GenParams()
+        lib.asc_gen_params_init(C.byref(p))
+        p.type, p.scale, p.num_vertices, p.num_samples = type, scale, num_vertices, num_samples
+        p.edge_factor, p.a, p.b, p.c, p.seed = edge_factor, a, b, c, seed
+        p.symmetrize, p.weighted, p.wmin, p.wmax = int(symmetrize), 1, wmin, wmax
+        p.offset_bytes = offset_bytes
+        h = C.c_void_p()
+        check(lib.asc_graph_generate(self._h, C.byref(p), C.byref(h)), "asc_graph_generate")
         return Graph(self, h)
 
     def from_arrays(self, off, edges, kind=None, out_degree=None):

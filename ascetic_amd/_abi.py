@@ -59,7 +59,7 @@ class Config(C.Structure):
         ("stream_overlap", C.c_int),
         ("bfs_persistent", C.c_int),
         ("release_host_edges", C.c_int),
-        ("pad_cfg", C.c_int),
+        ("part_buffers", C.c_int),
     ]
 
 
@@ -304,6 +304,7 @@ SIGNATURES = {
     "asc_host_csr_free": (None, [C.POINTER(HostCsr)]),
     "asc_host_csr_view": (None, [C.POINTER(HostCsr), C.POINTER(CsrView)]),
     "asc_graph_adopt": (C.c_int, [P, C.POINTER(HostCsr), C.c_int, PP]),
+    "asc_graph_generate": (C.c_int, [P, C.POINTER(GenParams), PP]),
     "asc_server_opts_init": (None, [C.POINTER(ServerOpts)]),
     "asc_server_create": (C.c_int, [P, C.POINTER(ServerOpts), PP]),
     "asc_server_destroy": (C.c_int, [P]),

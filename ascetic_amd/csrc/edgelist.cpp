@@ -19,22 +19,11 @@
 #include <algorithm>
 #include <cstring>
 
+#include "gen_sampler.hpp"
 #include "internal.hpp"
 
 namespace asc {
 namespace {
-
-inline uint64_t mix64(uint64_t z) {
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
-    return z ^ (z >> 31);
-}
-
-inline uint32_t pair_weight(uint64_t key, uint32_t a, uint32_t b, uint32_t wmin, uint32_t span) {
-    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    const uint64_t h = mix64(key ^ (((uint64_t)lo << 32) | hi));
-    return wmin + (uint32_t)(((unsigned __int128)h * span) >> 64);
-}
 
 struct Parsed {
     std::vector<uint32_t> u, v, w;   // w empty when no line had a weight
@@ -354,7 +343,7 @@ int read_edge_list(const char* path, const asc_edge_list_opts* o, HostCSR& out) 
             ASC_TRY(out.edges.alloc(2 * E + 1));
             const bool given = !sw.empty();
             const uint32_t span = o->wmax - o->wmin + 1;
-            const uint64_t wkey = mix64(o->seed ^ 0x3E16A7ull);
+            const uint64_t wkey = weight_key(o->seed);
             pool.parallel_for(V, [&](uint64_t b, uint64_t e) {
                 for (uint64_t x = b; x < e; x++)
                     for (uint64_t k = off[x]; k < off[x + 1]; k++) {

@@ -34,7 +34,7 @@ void graph_free_device(asc_graph* g) {
     g->pk_split = false;
 }
 
-static int param_size_for(int kind) {
+int param_size_for(int kind) {
     // paramSize of CalculateOpt.cu:20 (bfs), :389 (sssp), :552 (pr)
     return kind == ASC_KIND_WCSR ? 15 : kind == ASC_KIND_CSC ? 17 : 13;
 }
@@ -191,7 +191,7 @@ int graph_upload(asc_graph* g) {
     return ASC_OK;
 }
 
-static void compute_max_degree(asc_graph* g) {
+void compute_max_degree(asc_graph* g) {
     uint32_t best = 0, at = 0;
     for (uint64_t v = 0; v < g->h.V; v++) {
         uint64_t d = g->h.off[v + 1] - g->h.off[v];

@@ -262,4 +262,6 @@ void trav_free(TravState* t);
 void pr_free(PrState* p);
 int effective_heavy_threshold(const asc_graph* g);
 int check_symmetric(asc_graph* g);   // sets g->symmetric from the device copy
+int param_size_for(int kind);         // the reference's paramSize per graph kind (sizing rule)
+void compute_max_degree(asc_graph* g);
 }  // namespace asc

@@ -457,12 +457,12 @@ void trav_free(TravState* t) {
     for (int i = 0; i < 2; i++) {
         f(t->q[i]); f(t->pend[i]); f(t->far[i]);
     }
-    for (int i = 0; i < kPartBufs; i++) f(t->part[i]);
+    for (int i = 0; i < kMaxPartBufs; i++) f(t->part[i]);
     f(t->heavy_u); f(t->heavy_sv); f(t->heavy_base);
     f(t->od_list); f(t->od_prefix); f(t->scan_tmp); f(t->slots); f(t->piles); f(t->stats);
     f(t->bm[0]); f(t->bm[1]); f(t->bm[2]); f(t->pctl); f(t->dcnt); f(t->d_ch_v); f(t->d_ch_edges); f(t->d_ch_count); f(t->d_ch_dense);
     f(t->d_cursor);
-    for (int i = 0; i < kPartBufs; i++) {
+    for (int i = 0; i < kMaxPartBufs; i++) {
         if (t->ev_copy[i]) (void)hipEventDestroy(t->ev_copy[i]);
         if (t->ev_kern[i]) (void)hipEventDestroy(t->ev_kern[i]);
     }
@@ -554,7 +554,9 @@ int trav_ensure(asc_graph* g) {
         ASC_TRY(t->h_list.alloc(2 * nod, true));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
-        for (int i = 0; i < kPartBufs; i++) {
+        const int nb_cfg = g->ctx->cfg.part_buffers;
+        t->n_bufs = nb_cfg > 0 ? std::min(std::max(nb_cfg, 2), kMaxPartBufs) : kDefaultPartBufs;
+        for (int i = 0; i < t->n_bufs; i++) {
             t->buf_chunk[i] = -1;
             if (g->ctx->cfg.stream_mode != ASC_STREAM_ZEROCOPY)
                 ASC_HIP(hipMalloc(&t->part[i], EB * t->part_edges));
@@ -711,14 +713,14 @@ static const char* tail_w8(const asc_graph* g, uint64_t e) {
 // by an earlier round of the run and not overwritten since) reads it there.
 static int prefetch_chunks(asc_graph* g, TravState* t) {
     asc_ctx* ctx = g->ctx;
-    for (int b = 0; b < kPartBufs; b++) {
+    for (int b = 0; b < t->n_bufs; b++) {
         t->buf_chunk[b] = -1;
         t->buf_counted[b] = false;
     }
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
     if (!n_ch || ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY || ctx->cfg.dense_threshold > 1.0) return ASC_OK;
     if (!g->packed && !g->zc_tail.p) return ASC_OK;
-    for (int b = 0; b < kPartBufs && (uint32_t)b < n_ch; b++) {
+    for (int b = 0; b < t->n_bufs && (uint32_t)b < n_ch; b++) {
         const uint64_t e0 = t->ch_e[b], e1 = t->ch_e[b + 1];
         if (e1 - e0 > t->part_edges) break;
         ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
@@ -769,7 +771,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_HIP(hipMemcpyAsync(t->h_ch_edges.p, t->d_ch_edges, 8 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipMemcpyAsync(t->h_ch_count.p, t->d_ch_count, 4 * (size_t)n_ch, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
-        for (int b = 0; b < kPartBufs; b++)
+        for (int b = 0; b < t->n_bufs; b++)
             if (t->buf_chunk[b] >= 0 && (uint32_t)t->buf_chunk[b] < n_ch) buf_of[t->buf_chunk[b]] = b;
         if (ctx->cfg.dense_threshold > 0) {
             const double thr = ctx->cfg.dense_threshold;
@@ -960,7 +962,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             } else {
                 // the chunks read in place were all launched already, so any
                 // buffer may be overwritten once its last reader is done
-                b = (int)(nb++ % kPartBufs);
+                b = (int)(nb++ % t->n_bufs);
                 ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
                 ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
                 ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
@@ -1002,7 +1004,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
             const int sb = (int)(q & 1);
-            const int b = (int)(nb++ % kPartBufs);
+            const int b = (int)(nb++ % t->n_bufs);
             const uint64_t lo = q * PE, hi = std::min(Ts, lo + PE);
             if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();

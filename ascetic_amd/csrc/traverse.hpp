@@ -15,7 +15,8 @@ constexpr uint32_t kHeavyCap = 1u << 20;        // heavy list entries per level
 constexpr int kTile = 1024;                     // edges per tile in edge-balanced kernels
 constexpr int kMinHeavy = 256;                  // smallest heavy threshold (tile spans <= 5 entries)
 constexpr int kBlocksPerCU = 8;
-constexpr int kPartBufs = 4;                  // device buffers for streamed parts and chunks (round-robin)
+constexpr int kMaxPartBufs = 16;              // device buffers for streamed parts and chunks (round-robin)
+constexpr int kDefaultPartBufs = 4;
 constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
 constexpr uint32_t kMaxChunks = 1024;           // dense on-demand chunks (LDS histogram size)
 
@@ -139,13 +140,14 @@ struct TravState {
     Slot* slots = nullptr;
     Piles* piles = nullptr;
     DevStats* stats = nullptr;
-    void* part[kPartBufs] = {};     // device part buffers, used round-robin
+    void* part[kMaxPartBufs] = {};  // device part buffers, used round-robin (n_bufs of them)
+    int n_bufs = kDefaultPartBufs;
     // On-demand chunk whose edges part buffer b holds whole (-1: none), copied
     // ahead at the start of the run or by an earlier round of it; a round
     // that needs the chunk reads it there instead of copying it again.
     // buf_counted[b]: that copy already counts in the run's streamed edges.
-    int buf_chunk[kPartBufs] = {};
-    bool buf_counted[kPartBufs] = {};
+    int buf_chunk[kMaxPartBufs] = {};
+    bool buf_counted[kMaxPartBufs] = {};
     uint64_t part_edges = 0;
     HostBuf<Slot> h_slot;
     HostBuf<Piles> h_piles;
@@ -179,7 +181,7 @@ struct TravState {
     HostBuf<uint32_t> h_sample;
     // device part buffer b: copy into it done / last kernel reading it done;
     // staging buffer i: copy out of it done
-    hipEvent_t ev_copy[kPartBufs] = {}, ev_kern[kPartBufs] = {}, ev_meta = nullptr;
+    hipEvent_t ev_copy[kMaxPartBufs] = {}, ev_kern[kMaxPartBufs] = {}, ev_meta = nullptr;
     hipEvent_t ev_stage[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> ev_pool;
     // event pairs recorded during the current run: (kernel kind, start, end)

@@ -635,10 +635,10 @@ def c4_shape_section(A, _abi, O, args, cpu_threads):
     src = int(np.argmax(deg))
     log(f"c4-shape graph: R-MAT s{sc} ef{ef} w1-255 V={V} E={E} ({time.perf_counter() - t:.1f}s)")
     budget = int(args.c4_budget_frac * 8 * E)
-    out = {"workload": f"rmat_s{sc}_ef{ef}_w1-255_budget{int(round(args.c4_budget_frac * 100))}",
-           "graph": f"R-MAT scale {sc}, edge factor {ef}, (0.57,0.19,0.19,0.05), symmetrized, weights uniform in "
-                    f"[1, 255], seed {args.seed} (the scale-30 configuration's shape at the largest scale one "
-                    f"benchmark run generates and holds)",
+    out = {"workload": f"rmat_s{sc}_ef{2 * ef}directed_w1-255_budget{int(round(args.c4_budget_frac * 100))}",
+           "graph": f"R-MAT scale {sc}, {ef} undirected samples per vertex, symmetrized ({2 * ef} directed edges per "
+                    f"vertex), (0.57,0.19,0.19,0.05), weights uniform in [1, 255], seed {args.seed} (the scale-30 "
+                    f"configuration's shape at a scale whose host copy the oracle can check bit for bit)",
            "V": V, "E_directed": E, "source": src, "edge_budget_bytes": budget}
     # the library copies the arrays and, once the on-demand edges sit in their
     # packed pinned form, frees its {dst, w} copy (release_host_edges): the
@@ -691,6 +691,99 @@ def c4_shape_section(A, _abi, O, args, cpu_threads):
     return out
 
 
+def c4_full_section(A, _abi, O, args, shape):
+    """BFS and SSSP at the out-of-memory configuration itself: R-MAT at
+    args.c4_full_scale (30: 2^30 vertices and 16 undirected samples per
+    vertex, symmetrized, i.e. 32 directed edges per vertex, ~34 G directed
+    edges and 275 GB of {dst, w} records), weights uniform in [1, 255], a
+    96 GB device edge budget (scaled by 2^(scale - 30) below 30), source =
+    the highest-degree vertex.  The graph is generated straight into the
+    device's static prefix and the pinned packed tail (asc_graph_generate):
+    no host copy of its records exists, so parity is checked by properties
+    on 2^24 sampled edges drawn by the oracle's restatement of the sampler
+    (orc_rmat_pairs): BFS levels differ by at most one across every sampled
+    edge, SSSP distances satisfy d[v] <= d[u] + w(u, v) both ways, BFS and
+    SSSP reach the same vertices, and the source holds 1.  The scale-26
+    section of the same shape checks bit-exact parity with the oracle."""
+    sc = args.c4_full_scale
+    budget = int(args.c4_budget_gb * 1e9 * 2.0 ** min(0, sc - 30))
+    t = time.perf_counter()
+    with A.Context(edge_budget_bytes=budget, part_bytes=args.part_mb << 20, profile=True) as ctx:
+        g = ctx.generate_graph(type=0, scale=sc, edge_factor=16, seed=args.seed, wmin=1, wmax=255, offset_bytes=8)
+        gen_s = time.perf_counter() - t
+        info = g.info()
+        V, E, src = info["num_vertices"], info["num_edges"], info["max_degree_vertex"]
+        log(f"c4 graph: R-MAT s{sc} 32 directed edges per vertex, w1-255: V={V} E={E}, static "
+            f"{info['static_edges']} ({gen_s:.1f}s, host {info['host_bytes'] / 1e9:.1f} GB, "
+            f"device {info['device_bytes'] / 1e9:.1f} GB)")
+        out = {"workload": f"rmat_s{sc}_ef32directed_w1-255_budget{budget / 1e9:.0f}GB",
+               "graph": f"R-MAT scale {sc}, 16 undirected samples per vertex, symmetrized (32 directed edges per "
+                        f"vertex), (0.57,0.19,0.19,0.05), self-loops and duplicates removed, weights uniform in "
+                        f"[1, 255], seed {args.seed}, generated on the GPU straight into the device prefix and the "
+                        f"pinned tail",
+               "V": V, "E_directed": E, "source": src, "edge_budget_bytes": budget,
+               "static_edges": info["static_edges"], "ondemand_edges": info["ondemand_edges"],
+               "edge_record_bytes": 8 * E, "library_host_bytes": info["host_bytes"],
+               "device_bytes": info["device_bytes"], "generation_s": round(gen_s, 1)}
+        h2d, _ = ctx.measure_link(256 << 20, 8)
+        vals = {}
+        for algo in ("bfs", "sssp"):
+            fn = getattr(g, algo)
+            first, st = fn(src)
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            runs = [fn(src, output=False)[1] for _ in range(args.c4_full_runs)]
+            ctx.synchronize()
+            step_s = (time.perf_counter() - t0) / args.c4_full_runs
+            num = st["reached_degree_sum"] / 2
+            sb = float(np.mean([r["streamed_bytes"] for r in runs]))
+            out[algo] = {"gteps": round(num / step_s / 1e9, 4), "ms_per_run": round(step_s * 1e3, 1),
+                         "runs": args.c4_full_runs, "iterations": st["iterations"], "reached": st["reached"],
+                         "stream_record_bytes": round(sb / max(1.0, float(np.mean([r["streamed_edges"]
+                                                                                    for r in runs])))),
+                         "link": {"streamed_GB_per_run": round(sb / 1e9, 2),
+                                  "achieved_GBps": round(sb / step_s / 1e9, 2), "peak_h2d_GBps": round(h2d, 2),
+                                  "frac": round(sb / step_s / 1e9 / h2d, 4) if h2d else None,
+                                  "gather_ms_per_run": round(float(np.mean([r["fill_ms"] for r in runs])), 1)},
+                         "kernels": _kernel_table(runs[-1], 8, 8, V)}
+            vals[algo] = first
+            log(f"c4 {algo}: {out[algo]['gteps']} GTEPS, {out[algo]['ms_per_run']} ms/run, "
+                f"link {out[algo]['link']['achieved_GBps']} GB/s")
+        g.close()
+    # properties on sampled edges
+    t = time.perf_counter()
+    u, v, w = O.rmat_pairs(args.seed, sc, 16 << sc, 1 << 24, wmin=1, wmax=255)
+    keep = u != v
+    u, v, w = u[keep], v[keep], w[keep]
+    lv, d = vals["bfs"], vals["sssp"]
+    unreached = np.uint32(V + 1)
+    lu, lw = lv[u].astype(np.int64), lv[v].astype(np.int64)
+    du, dv = d[u].astype(np.int64), d[v].astype(np.int64)
+    checks = {
+        "source_holds_1": bool(lv[src] == 1 and d[src] == 1),
+        "same_reached_set": bool(np.array_equal(lv == unreached, d == unreached)),
+        "reached_count": bool(int((lv != unreached).sum()) == out["bfs"]["reached"] == out["sssp"]["reached"]),
+        "bfs_levels_within_one": bool(np.all((lu == V + 1) == (lw == V + 1)) and
+                                      np.all(np.abs(lu - lw)[lu != V + 1] <= 1)),
+        "sssp_edge_inequality": bool(np.all(((du == V + 1) & (dv == V + 1)) |
+                                            ((du != V + 1) & (dv != V + 1) & (dv <= du + w) & (du <= dv + w)))),
+    }
+    out["parity"] = {"sampled_edges": int(keep.sum()), "checks": checks,
+                     "ok": all(checks.values()), "seconds": round(time.perf_counter() - t, 1),
+                     "exact_parity": "bit-exact vs the oracle at scale 26 of the same shape (c4_shape)"}
+    log(f"c4 properties: {checks}")
+    if shape and "sssp" in shape:
+        cb = shape["sssp"]["cpu_baseline"]
+        out["cpu_baseline"] = {"value": cb["value"], "unit": "GTEPS", "cores": cb["cores"], "kind": "port",
+                               "sample": "the oracle's restatement of sssp_opt timed on the scale-26 graph of the same "
+                                         "shape (c4_shape); the scale-30 run has no host copy of its records to "
+                                         "time it on",
+                               "edges_relaxed_per_s": cb["edges_relaxed_per_s"],
+                               "single_thread": cb["single_thread"]}
+    del vals, lv, d
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -727,9 +820,16 @@ def main():
                     help="PageRank schedule of the per-algorithm line")
     ap.add_argument("--c4-scale", type=int, default=26,
                     help="R-MAT scale of the section at the out-of-memory configuration's shape (0 = skip)")
-    ap.add_argument("--c4-edge-factor", type=int, default=32)
-    ap.add_argument("--c4-budget-frac", type=float, default=0.35,
-                    help="device edge budget of that section as a fraction of its edge+weight bytes")
+    ap.add_argument("--c4-edge-factor", type=int, default=16,
+                    help="undirected samples per vertex (symmetrized: twice as many directed edges)")
+    ap.add_argument("--c4-budget-frac", type=float, default=0.355,
+                    help="device edge budget of that section as a fraction of its edge+weight bytes "
+                         "(96 GB of 270 GB at scale 30)")
+    ap.add_argument("--c4-full-scale", type=int, default=30,
+                    help="R-MAT scale of the out-of-memory configuration itself, generated on the GPU (0 = skip)")
+    ap.add_argument("--c4-budget-gb", type=float, default=96.0,
+                    help="device edge budget at scale 30 in GB (scaled by 2^(scale-30) below)")
+    ap.add_argument("--c4-full-runs", type=int, default=2, help="timed runs per algorithm at full scale")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -774,6 +874,8 @@ def main():
         hr = None
     if args.c4_scale > 0:
         result["c4_shape"] = c4_shape_section(A, _abi, O, args, cpu_threads)
+    if args.c4_full_scale > 0:
+        result["c4"] = c4_full_section(A, _abi, O, args, result.get("c4_shape"))
     print(json.dumps(result), flush=True)
     bad = [k for k, v in _parity_flags(result) if not v]
     if bad:
@@ -795,6 +897,8 @@ def _parity_flags(result):
         for k, v in result.get(sec, {}).items():
             if isinstance(v, dict) and "exact" in v:
                 flags.append((f"{sec}.{k}", v["exact"]))
+    if "c4" in result:
+        flags.append(("c4.properties", result["c4"]["parity"]["ok"]))
     sv = result.get("serving", {})
     if sv:
         flags.append(("serving.exact_sample", not sv.get("exact_sample", "").startswith("MISMATCH")))

@@ -174,7 +174,11 @@ typedef struct {
                                     then no longer needed: the device holds
                                     the static prefix); the graph can then not
                                     be re-planned.  0 (default): keep it     */
-    int pad_cfg;
+    int part_buffers;            /* device buffers for streamed parts and
+                                    chunks, each part_bytes (0 = 4; 2..16):
+                                    more let more of the on-demand region be
+                                    copied while the resident part is relaxed
+                                    at the start of a run                    */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -463,6 +467,19 @@ void asc_host_csr_view(const asc_host_csr* h, asc_csr_view* v);
 /* Builds a graph directly from a generated CSR, taking ownership of its arrays
  * (no copy; *h is emptied). */
 int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph** out);
+/* Generates a weighted graph (the CSR asc_generate builds for the same
+ * parameters) straight into a device graph: the static prefix the context's
+ * budget allows is written on the GPU, the on-demand edges go to pinned host
+ * memory in their packed 4-byte or split 5-byte form, and the host never
+ * holds the 8-byte {dst, w} records; at R-MAT scale 30 with edge factor 16
+ * (34 G directed edges) that is 8 GB of offsets and ~110 GB of records in
+ * host memory instead of 275 GB.  Two sweeps over vertex ranges on the GPU
+ * (degrees, then records); p->offset_bytes = 8 keeps 8-byte device offsets
+ * (forced when E >= 2^32).  The graph cannot be re-planned (its host records
+ * do not exist).  ASC_ERR_UNSUPPORTED for unweighted or in-CSR parameters,
+ * or with stream_pack 0.  The reference reads its large graphs from files
+ * (GraphMeta.cuh:151-171); this replaces writing and reading one. */
+int asc_graph_generate(asc_ctx* ctx, const asc_gen_params* p, asc_graph** out);
 
 /* ---- batched query serving --------------------------------------------- */
 /* No counterpart in the reference (which runs one traversal per process).

@@ -361,6 +361,43 @@ int orc_pr_norm(const orc_graph* g, const uint32_t* outdeg, double damping, int 
     return 0;
 }
 
+/* ---- sampled edges of generated graphs --------------------------------- */
+static uint64_t sm_finalise(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+static uint32_t prob_threshold(double x) {
+    double t = x * 4294967296.0;
+    return t >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)t;
+}
+
+int orc_rmat_pairs(uint64_t seed, int scale, double a, double b, double c, uint64_t M, uint32_t wmin,
+                   uint32_t wmax, uint64_t k, uint32_t* u, uint32_t* v, uint32_t* w) {
+    if (scale < 1 || scale > 31 || wmax < wmin) return -1;
+    const uint64_t key = sm_finalise(seed ^ 0x5A17EC0DEull), wkey = sm_finalise(seed ^ 0x3E16A7ull);
+    const uint32_t ta = prob_threshold(a), tab = prob_threshold(a + b), tabc = prob_threshold(a + b + c);
+    const uint64_t span = (uint64_t)wmax - wmin + 1;
+#pragma omp parallel for schedule(static)
+    for (uint64_t j = 0; j < k; j++) {
+        const uint64_t i = (uint64_t)(((unsigned __int128)j * M) / k);
+        uint32_t x = 0, y = 0;
+        uint64_t word = 0;
+        for (int l = 0; l < scale; l++) {
+            if ((l & 1) == 0) word = sm_finalise(key + ((i << 5) + (uint64_t)(l >> 1)) * 0x9E3779B97F4A7C15ull);
+            const uint32_t r = (uint32_t)(word >> ((l & 1) * 32));
+            x = (x << 1) | (uint32_t)(r >= tab);
+            y = (y << 1) | (uint32_t)((r >= ta && r < tab) || r >= tabc);
+        }
+        const uint32_t lo = x < y ? x : y, hi = x < y ? y : x;
+        const uint64_t h = sm_finalise(wkey ^ (((uint64_t)lo << 32) | hi));
+        u[j] = x;
+        v[j] = y;
+        w[j] = wmin + (uint32_t)(((unsigned __int128)h * span) >> 64);
+    }
+    return 0;
+}
+
 /* fillEdgeArrByMultiThread (GraphMeta.cuh:469-514): the adjacency of each
  * listed vertex copied, in list order, into one contiguous buffer; the list
  * is cut into `nthreads` chunks of ceil(n / nthreads) vertices (one thread

@@ -125,6 +125,16 @@ uint64_t orc_last_pulled(void);
 long long orc_gather_ref(const orc_graph* g, const uint32_t* list, uint64_t n, int nthreads, uint32_t* out,
                          double* copy_ms);
 
+/* Sampled edges of a synthetic R-MAT graph too large for a host copy: the
+ * library's sampler (a SplitMix64 finaliser over (seed, sample index), two
+ * 32-bit quadrant draws per 64-bit word, weights per unordered pair) restated
+ * independently, so that a test can draw edges of the generated graph and
+ * check properties of a traversal's values on them.  Writes the pair of
+ * sample i_j = floor(j * M / k) for j < k (u == v for a dropped self-loop)
+ * and its weight in [wmin, wmax]. */
+int orc_rmat_pairs(uint64_t seed, int scale, double a, double b, double c, uint64_t M, uint32_t wmin,
+                   uint32_t wmax, uint64_t k, uint32_t* u, uint32_t* v, uint32_t* w);
+
 /* Fault injection for the oracle's own tests: 0 = none; any other value
  * plants one deliberate mistake (listed in oracle.c) so a test can show the
  * checks catch it. */

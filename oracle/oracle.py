@@ -71,6 +71,8 @@ def load():
             "orc_last_pulled": [],
             "orc_set_num_threads": [C.c_int],
             "orc_gather_ref": [G, u32p, C.c_uint64, C.c_int, u32p, f64p],
+            "orc_rmat_pairs": [C.c_uint64, C.c_int, C.c_double, C.c_double, C.c_double, C.c_uint64, C.c_uint32,
+                               C.c_uint32, C.c_uint64, u32p, u32p, u32p],
         }
         void = ("orc_set_fault", "orc_set_iteration_limit", "orc_set_num_threads", "orc_set_edge_limit")
         for name, args in sig.items():
@@ -252,6 +254,19 @@ class edge_limit:
 
     def __exit__(self, *exc):
         load().orc_set_edge_limit(0)
+
+
+def rmat_pairs(seed, scale, M, k, wmin=1, wmax=64, a=0.57, b=0.19, c=0.19):
+    """Pairs and weights of k samples spread over the M samples of the
+    library's R-MAT generator (u == v for a self-loop sample)."""
+    u = np.empty(k, dtype=np.uint32)
+    v = np.empty(k, dtype=np.uint32)
+    w = np.empty(k, dtype=np.uint32)
+    r = load().orc_rmat_pairs(seed, scale, a, b, c, M, wmin, wmax, k, u.ctypes.data_as(u32p),
+                              v.ctypes.data_as(u32p), w.ctypes.data_as(u32p))
+    if r != 0:
+        raise ValueError("orc_rmat_pairs: bad parameters")
+    return u, v, w
 
 
 def last_pulled():
