@@ -210,7 +210,7 @@ class Context:
                  gather_threads=0, stream_mode=_abi.STREAM_GATHER, value_convention=_abi.VALUES_REFERENCE,
                  heavy_threshold=0, pr_group=0, sssp_delta=0, verbose=False, profile=False, stream_pack=1,
                  bfs_direction=_abi.BFS_AUTO, cc_method=_abi.CC_AUTO, dense_threshold=0.0, stream_overlap=None,
-                 bfs_persistent=False, release_host_edges=False, part_buffers=0):
+                 bfs_persistent=False, release_host_edges=False, part_buffers=0, max_pinned_bytes=0):
         lib = _abi.load()
         cfg = _abi.Config()
         lib.asc_config_init(C.byref(cfg))
@@ -228,6 +228,7 @@ class Context:
         cfg.bfs_persistent = int(bool(bfs_persistent))
         cfg.release_host_edges = int(bool(release_host_edges))
         cfg.part_buffers = int(part_buffers)
+        cfg.max_pinned_bytes = int(max_pinned_bytes)
         h = C.c_void_p()
         check(lib.asc_ctx_create(C.byref(cfg), C.byref(h)), "asc_ctx_create")
         self._h = h

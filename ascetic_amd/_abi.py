@@ -60,6 +60,8 @@ class Config(C.Structure):
         ("bfs_persistent", C.c_int),
         ("release_host_edges", C.c_int),
         ("part_buffers", C.c_int),
+        ("pad_cfg", C.c_int),
+        ("max_pinned_bytes", C.c_uint64),
     ]
 
 
@@ -90,7 +92,7 @@ class GraphInfo(C.Structure):
         ("max_degree", C.c_uint32),
         ("max_degree_vertex", C.c_uint32),
         ("symmetric", C.c_int),
-        ("pad", C.c_int),
+        ("tail_pinned", C.c_int),
         ("host_bytes", C.c_uint64),
     ]
 

@@ -135,7 +135,7 @@ int graph_upload(asc_graph* g) {
         const uint32_t wbits = span ? 32 - (uint32_t)__builtin_clz(span) : 0;
         const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
         if (dbits + wbits <= 32 && dbits < 32 && ctx->cfg.stream_pack != 2) {
-            ASC_TRY(g->ptail.alloc(n, true));
+            ASC_TRY(g->ptail.alloc_prefer_pinned(n, ctx->cfg.max_pinned_bytes));
             host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
                 for (uint64_t i = b; i < en; i++)
                     g->ptail[i] = ew[2 * (S + i)] | ((ew[2 * (S + i) + 1] - wlo) << dbits);
@@ -145,8 +145,8 @@ int graph_upload(asc_graph* g) {
             g->pk_wmin = wlo;
             g->part_edges = std::max<uint64_t>(pb / 4, 1024);
         } else if (wbits <= 8) {
-            ASC_TRY(g->ptail.alloc(n, true));
-            ASC_TRY(g->ptail_w.alloc(n, true));
+            ASC_TRY(g->ptail.alloc_prefer_pinned(n, ctx->cfg.max_pinned_bytes));
+            ASC_TRY(g->ptail_w.alloc_prefer_pinned(n, ctx->cfg.max_pinned_bytes));
             host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
                 for (uint64_t i = b; i < en; i++) {
                     g->ptail[i] = ew[2 * (S + i)];
@@ -427,6 +427,7 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info) {
     info->max_degree = g->max_degree;
     info->max_degree_vertex = g->max_degree_vertex;
     info->symmetric = g->symmetric;
+    info->tail_pinned = g->static_edges < g->h.E && g->tail_pinned() ? 1 : 0;
     info->host_bytes = 8 * (g->h.V + 1) + (g->h.edges.p ? (uint64_t)g->h.E * g->h.edge_bytes() : 0) +
                        4ull * g->h.outdeg.n + 4ull * g->ptail.n + g->ptail_w.n + 4ull * g->zc_tail.n;
     return ASC_OK;

@@ -491,8 +491,8 @@ int graph_generate(asc_ctx* ctx, const asc_gen_params* p, asc_graph** out) {
     }
     g->device_bytes = ob * (V + 1) + sbytes;
     if (n_tail) {
-        ASC_TRY(g->ptail.alloc(n_tail, true));
-        if (split) ASC_TRY(g->ptail_w.alloc(n_tail, true));
+        ASC_TRY(g->ptail.alloc_prefer_pinned(n_tail, ctx->cfg.max_pinned_bytes));
+        if (split) ASC_TRY(g->ptail_w.alloc_prefer_pinned(n_tail, ctx->cfg.max_pinned_bytes));
         const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
         g->packed = true;
         g->pk_split = split;

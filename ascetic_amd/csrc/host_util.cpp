@@ -136,6 +136,8 @@ void asc_config_init(asc_config* c) {
     c->bfs_persistent = 0;
     c->release_host_edges = 0;
     c->part_buffers = 0;
+    c->pad_cfg = 0;
+    c->max_pinned_bytes = 0;
 }
 
 void asc_pr_opts_init(asc_pr_opts* o) {

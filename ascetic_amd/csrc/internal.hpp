@@ -72,6 +72,7 @@ struct HostBuf {
         if (count == 0) return ASC_OK;
         if (pin) {
             if (hipHostMalloc((void**)&p, count * sizeof(T), hipHostMallocDefault) != hipSuccess) {
+                (void)hipGetLastError();   // reported here; later error checks must not see it
                 p = nullptr;
                 set_last_error("hipHostMalloc of %zu bytes failed", count * sizeof(T));
                 return ASC_ERR_NOMEM;
@@ -86,6 +87,14 @@ struct HostBuf {
         n = count;
         pinned = pin;
         return ASC_OK;
+    }
+    // Page-locked when the system grants it (page-locked memory is capped
+    // well below the host's RAM: about 70 GB of 300 on the benchmark hosts),
+    // else pageable.
+    // max_bytes > 0: page-locked only up to that size.
+    int alloc_prefer_pinned(size_t count, uint64_t max_bytes) {
+        if ((!max_bytes || count * sizeof(T) <= max_bytes) && alloc(count, true) == ASC_OK) return ASC_OK;
+        return alloc(count, false);
     }
     void release() {
         if (p) {
@@ -232,6 +241,11 @@ struct asc_graph {
     uint32_t pk_shift = 32, pk_wmin = 0;
     // bytes per streamed on-demand edge
     size_t stream_record_bytes() const { return packed ? (pk_split ? 5 : 4) : h.edge_bytes(); }
+    // the on-demand records sit in page-locked memory (DMA and zero-copy
+    // reads straight from them); pageable packed tails go through staging
+    bool tail_pinned() const {
+        return packed ? ptail.pinned && (!pk_split || ptail_w.pinned) : zc_tail.p != nullptr;
+    }
     // tunables
     int heavy_threshold = 0;
     int pr_group = 0;

@@ -524,6 +524,12 @@ int trav_ensure(asc_graph* g) {
     const size_t EB = g->stream_record_bytes();  // bytes per streamed edge
     TravState* t = g->trav;
     const bool need_parts = g->static_vertices < V;
+    const int mode = g->ctx->cfg.stream_mode;
+    if (need_parts && (mode == ASC_STREAM_ZEROCOPY || mode == ASC_STREAM_HYBRID) && !g->tail_pinned()) {
+        set_last_error("zero-copy and hybrid streaming read the on-demand edges from page-locked memory, which the "
+                       "system did not grant for this graph's tail; use gather streaming");
+        return ASC_ERR_UNSUPPORTED;
+    }
     if (t && t->V == V && (!need_parts || (t->od_list && t->part_edges >= g->part_edges))) return ASC_OK;
     if (t) trav_free(t);
     t = new TravState();
@@ -569,6 +575,8 @@ int trav_ensure(asc_graph* g) {
             if (g->ctx->cfg.stream_mode == ASC_STREAM_GATHER)
                 ASC_TRY(t->stage[i].alloc(EB * t->part_edges, true));
             ASC_HIP(hipEventCreateWithFlags(&t->ev_stage[i], hipEventDisableTiming));
+            // recorded once so that every use can wait for the buffer's last copy
+            ASC_HIP(hipEventRecord(t->ev_stage[i], g->ctx->s_copy));
         }
         ASC_HIP(hipEventCreateWithFlags(&t->ev_meta, hipEventDisableTiming));
         ASC_TRY(plan_dense_chunks(g, t));
@@ -700,6 +708,37 @@ static const char* tail_w8(const asc_graph* g, uint64_t e) {
     return g->pk_split ? (const char*)(g->ptail_w.p + (e - g->static_edges)) : nullptr;
 }
 
+// Queues the copy of the on-demand edges [e0, e1) (a whole chunk) into part
+// buffer b on stream s: one DMA straight from the tail when it is
+// page-locked; from a pageable tail the host threads first copy the records
+// into the next pinned staging buffer (the gather's, taken in turn), which
+// then goes by DMA.  *fill_ms accumulates the host copy time.
+static int copy_chunk_to_part(asc_graph* g, TravState* t, int b, uint64_t e0, uint64_t e1, hipStream_t s,
+                              double* fill_ms) {
+    if (g->tail_pinned()) return copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, s);
+    if (!t->stage[0].p) {
+        set_last_error("a pageable on-demand tail needs the gather streaming mode's staging buffers");
+        return ASC_ERR_UNSUPPORTED;
+    }
+    const int sb = (int)(t->stage_seq++ & 1);
+    ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));   // its previous copy has left
+    const uint64_t n = e1 - e0;
+    const size_t rb = g->pk_split ? 4 : g->stream_record_bytes();
+    char* dst = t->stage[sb].p;
+    char* dst_w = dst + 4 * t->part_edges;
+    const char* src = tail_rec(g, e0);
+    const char* src_w = tail_w8(g, e0);
+    const double t0 = now_ms();
+    g->ctx->pool->parallel_for(n, [&](uint64_t lo, uint64_t hi) {
+        std::memcpy(dst + lo * rb, src + lo * rb, (hi - lo) * rb);
+        if (src_w) std::memcpy(dst_w + lo, src_w + lo, hi - lo);
+    });
+    if (fill_ms) *fill_ms += now_ms() - t0;
+    ASC_TRY(copy_to_part(g, t, b, dst, dst_w, n, s));
+    ASC_HIP(hipEventRecord(t->ev_stage[sb], s));
+    return ASC_OK;
+}
+
 // Streams one pile: filter on the device, list to the host, u64 prefix of
 // degrees, then parts (gather + double-buffered copies) or zero-copy reads.
 // With dense chunks planned, the pile's active edges are first counted per
@@ -724,7 +763,7 @@ static int prefetch_chunks(asc_graph* g, TravState* t) {
         const uint64_t e0 = t->ch_e[b], e1 = t->ch_e[b + 1];
         if (e1 - e0 > t->part_edges) break;
         ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
-        ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
+        ASC_TRY(copy_chunk_to_part(g, t, b, e0, e1, ctx->s_copy, nullptr));
         ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
         t->buf_chunk[b] = b;
     }
@@ -932,6 +971,8 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         });
     };
     uint64_t dense_edges = 0;
+    double fill_round = 0;     // host copy time: staged chunks and the gather
+    double gather_round = 0;   // the gather alone (its rate estimate)
     if (ctx->cfg.stream_mode == ASC_STREAM_ZEROCOPY) {
         ASC_TRY(map_tail());
         p.lo = 0;
@@ -964,7 +1005,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                 // buffer may be overwritten once its last reader is done
                 b = (int)(nb++ % t->n_bufs);
                 ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
-                ASC_TRY(copy_to_part(g, t, b, tail_rec(g, e0), tail_w8(g, e0), e1 - e0, ctx->s_copy));
+                ASC_TRY(copy_chunk_to_part(g, t, b, e0, e1, ctx->s_copy, &fill_round));
                 ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
                 dense_edges += e1 - e0;
                 if (t->buf_chunk[b] >= 0) buf_of[t->buf_chunk[b]] = -1;
@@ -1000,17 +1041,18 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             return ASC_OK;
         }
         // the rest: host gather into pinned staging, double-buffered copies
-        double fill_round = 0;
         const uint64_t np = (Ts + PE - 1) / PE;
         for (uint64_t q = 0; q < np; q++) {
-            const int sb = (int)(q & 1);
+            const int sb = (int)(t->stage_seq++ & 1);
             const int b = (int)(nb++ % t->n_bufs);
             const uint64_t lo = q * PE, hi = std::min(Ts, lo + PE);
-            if (q >= 2) ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
+            ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
             char* stage_w = t->stage[sb].p + 4 * PE;   // split form: weight bytes behind the ids
             gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p, stage_w);
-            fill_round += now_ms() - tg;
+            const double dt = now_ms() - tg;
+            fill_round += dt;
+            gather_round += dt;
             ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
             t->buf_chunk[b] = -1;                                       // now holds gathered records
             ASC_TRY(copy_to_part(g, t, b, t->stage[sb].p, stage_w, hi - lo, ctx->s_copy));
@@ -1031,8 +1073,8 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         rc.parts += nb;
         rc.streamed_edges += Ts + dense_edges;
         rc.dense_edges += dense_edges;
-        if (Ts * EB >= (32u << 20) && fill_round > 0)   // running estimate of the gather rate
-            t->gather_bps = 0.5 * t->gather_bps + 0.5 * (double)(Ts * EB) / (fill_round * 1e-3);
+        if (Ts * EB >= (32u << 20) && gather_round > 0)   // running estimate of the gather rate
+            t->gather_bps = 0.5 * t->gather_bps + 0.5 * (double)(Ts * EB) / (gather_round * 1e-3);
     }
     rc.overload_ms += now_ms() - t_begin;
     if (ctx->cfg.verbose > 1)

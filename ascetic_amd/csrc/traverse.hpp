@@ -18,7 +18,7 @@ constexpr int kBlocksPerCU = 8;
 constexpr int kMaxPartBufs = 16;              // device buffers for streamed parts and chunks (round-robin)
 constexpr int kDefaultPartBufs = 4;
 constexpr uint32_t kNoValue = 0xFFFFFFFFu;      // `done` before a vertex was ever expanded
-constexpr uint32_t kMaxChunks = 1024;           // dense on-demand chunks (LDS histogram size)
+constexpr uint32_t kMaxChunks = 4096;           // dense on-demand chunks (LDS histogram size)
 
 // Per-level counters, in a ring of four: level L reads slot L%4, appends to
 // slot (L+1)%4 and clears slot (L+2)%4.
@@ -155,6 +155,7 @@ struct TravState {
     HostBuf<uint32_t> h_od, h_list;
     HostBuf<unsigned long long> h_prefix;
     HostBuf<char> stage[2];
+    uint64_t stage_seq = 0;         // uses of the two staging buffers (gather and staged chunk copies)
     // dense on-demand chunks: vertex ranges [ch_v[i], ch_v[i+1]) whose edges
     // [ch_e[i], ch_e[i+1]) fit one part buffer (empty: dense copies off)
     std::vector<uint32_t> ch_v;

@@ -179,6 +179,14 @@ typedef struct {
                                     more let more of the on-demand region be
                                     copied while the resident part is relaxed
                                     at the start of a run                    */
+    int pad_cfg;
+    uint64_t max_pinned_bytes;   /* largest page-locked on-demand tail (packed
+                                    records) a graph takes; a larger one, or
+                                    one the system refuses to lock (about 70
+                                    GB on a 300 GB host), lives in pageable
+                                    memory, and its dense chunks go through
+                                    pinned staging (gather streaming only).
+                                    0 = no limit                             */
 } asc_config;
 
 typedef struct asc_ctx asc_ctx;
@@ -238,7 +246,9 @@ typedef struct {
     int symmetric;               /* 1: every u->v has a v->u of the same
                                     multiplicity; 0: not; -1: not checked
                                     (checked on upload when fully resident)  */
-    int pad;
+    int tail_pinned;             /* 1: the on-demand records are page-locked
+                                    (DMA and zero-copy reads straight from
+                                    them); 0: pageable or no tail            */
     uint64_t host_bytes;         /* host memory the graph holds: offsets, the
                                     edge records unless released, pinned
                                     on-demand copies, out-degrees            */

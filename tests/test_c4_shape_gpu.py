@@ -137,3 +137,34 @@ def test_release_host_edges_after_packing(rmat_ef32, pack):
         with pytest.raises(A.AscError) as e:
             g.replan(edge_budget_bytes=8 * E // 2)
         assert e.value.code == _abi.ERR_UNSUPPORTED
+
+
+@pytest.mark.parametrize("pack,rec", [(2, 5), (1, 4)])
+def test_pageable_tail(rmat_ef32, pack, rec):
+    """A tail larger than max_pinned_bytes (as at scale 30, where the host
+    grants about 70 GB of page-locked memory for 110 GB of records) lives in
+    pageable memory: dense chunks go through pinned staging, results are the
+    oracle's; zero-copy and hybrid streaming, which need page-locked records,
+    are refused."""
+    off, ew, src, d_ref, b_ref, d7_ref = rmat_ef32
+    E = int(off[-1])
+    say("c4 shape pageable tail, pack", pack)
+    with A.Context(edge_budget_bytes=8 * E // 3, part_bytes=rec * 8192, stream_pack=pack,
+                   max_pinned_bytes=1) as ctx, ctx.from_arrays(off, ew) as g:
+        assert g.info()["tail_pinned"] == 0
+        for _ in range(2):   # the second run reuses chunks left in the part buffers
+            d, st = g.sssp(src)
+            assert np.array_equal(d, d_ref)
+            assert st["dense_edges"] > 0 and st["streamed_bytes"] == st["streamed_edges"] * rec
+        lv, _ = g.bfs(src)
+        assert np.array_equal(lv, b_ref)
+        d, _ = g.sssp(7)
+        assert np.array_equal(d, d7_ref)
+    with A.Context(edge_budget_bytes=8 * E // 3, part_bytes=rec * 8192, stream_pack=pack,
+                   max_pinned_bytes=0) as ctx, ctx.from_arrays(off, ew) as g:
+        assert g.info()["tail_pinned"] == 1
+    for mode in ("zerocopy", "hybrid"):
+        with A.Context(edge_budget_bytes=8 * E // 3, part_bytes=rec * 8192, stream_pack=pack, max_pinned_bytes=1,
+                       stream_mode=MODES[mode]) as ctx, ctx.from_arrays(off, ew) as g:
+            with pytest.raises(A.AscError):
+                g.sssp(src)

@@ -96,3 +96,18 @@ def test_generated_graph_refuses_replan_and_unpacked():
     with A.Context(edge_budget_bytes=1 << 18, stream_pack=0) as ctx:
         with pytest.raises(A.AscError):
             ctx.generate_graph(type=0, scale=12, edge_factor=16, seed=2, wmax=64)
+
+
+def test_generated_graph_pageable_tail(reference):
+    """The generator's tail in pageable memory (max_pinned_bytes below its
+    size): gather streaming through pinned staging gives the oracle's values."""
+    off, ew, src, b_ref, d_ref, c_ref = reference["rmat14_ef32_w255"]
+    E = int(off[-1])
+    say("graph-generate pageable tail")
+    with A.Context(edge_budget_bytes=8 * E // 3, part_bytes=5 * 8192, stream_pack=2, max_pinned_bytes=1) as ctx, \
+            ctx.generate_graph(**PARAMS["rmat14_ef32_w255"]) as g:
+        assert g.info()["tail_pinned"] == 0
+        lv, _ = g.bfs(src)
+        assert np.array_equal(lv, b_ref)
+        d, st = g.sssp(src)
+        assert np.array_equal(d, d_ref) and st["dense_edges"] > 0
