@@ -625,9 +625,11 @@ static uint32_t sssp_delta(asc_graph* g) {
     // streamed graphs take wide bands (each band re-streams the on-demand
     // vertices it improves, so fewer rounds win; streamed vertices are
     // expanded with their value at expansion time, which limits the
-    // re-expansions wide bands cause); fully resident ones narrow bands (fewer
-    // re-expansions; a step costs only a launch and a sync)
-    const double f = g->static_vertices == g->h.V ? 0.25 : 6.0;
+    // re-expansions wide bands cause; measured: R-MAT scale 24, weights 1-64,
+    // 6 to 30 times the mean weight equal, 2 times 5% slower; scale 28,
+    // weights 1-255, 12 times 7% faster than 6); fully resident ones narrow
+    // bands (fewer re-expansions; a step costs only a launch and a sync)
+    const double f = g->static_vertices == g->h.V ? 0.25 : 12.0;
     return (uint32_t)std::min(std::max(1.0, f * g->mean_weight), 2147483648.0);
 }
 
