@@ -41,25 +41,49 @@ __global__ __launch_bounds__(kGenBlock) void k_gen_count(Sampler s, uint64_t M, 
     }
 }
 
-// Appends key when want: one atomic per wave.  Keys past cap are counted
-// but not stored (the host then reports the mismatch).
-__device__ __forceinline__ void gen_append(bool want, unsigned long long key, unsigned long long* keys,
-                                           unsigned long long* count, unsigned long long cap) {
+// Keys are staged per workgroup in LDS and flushed to the global array with
+// one atomic per flush: a single global counter taking one atomic per wave
+// serialised the pass (measured: 6 s per pass over 2^34 samples, against
+// 2.5 s for the count pass).  Keys past cap are counted but not stored (the
+// host then reports the mismatch).
+constexpr uint32_t kGenStage = 4096;
+
+__device__ __forceinline__ void gen_stage(bool want, unsigned long long key, unsigned long long* buf,
+                                          uint32_t* n) {
     const uint64_t m = wave_ballot(want);
     if (!m) return;
     const uint32_t leader = (uint32_t)__builtin_ctzll(m);
-    unsigned long long base = 0;
-    if (lane_id() == leader) base = atomicAdd(count, (unsigned long long)__popcll(m));
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(n, (uint32_t)__popcll(m));
     base = __shfl(base, (int)leader, kWave);
-    const unsigned long long pos = base + lanes_below(m);
-    if (want && pos < cap) keys[pos] = key;
+    if (want) buf[base + lanes_below(m)] = key;
+}
+
+__device__ __forceinline__ void gen_flush(unsigned long long* buf, uint32_t* n, unsigned long long* s_base,
+                                          unsigned long long* keys, unsigned long long* count,
+                                          unsigned long long cap) {
+    __syncthreads();
+    const uint32_t cnt = *n;
+    if (threadIdx.x == 0) *s_base = cnt ? atomicAdd(count, (unsigned long long)cnt) : 0;
+    __syncthreads();
+    const unsigned long long base = *s_base;
+    for (uint32_t i = threadIdx.x; i < cnt; i += kGenBlock)
+        if (base + i < cap) keys[base + i] = buf[i];
+    __syncthreads();
+    if (threadIdx.x == 0) *n = 0;
+    __syncthreads();
 }
 
 __global__ __launch_bounds__(kGenBlock) void k_gen_keys(Sampler s, uint64_t M, int sym, uint32_t lo, uint32_t hi,
                                                         uint32_t vbits, unsigned long long* keys,
                                                         unsigned long long* count, unsigned long long cap) {
+    __shared__ unsigned long long buf[kGenStage];
+    __shared__ uint32_t n;
+    __shared__ unsigned long long s_base;
+    if (threadIdx.x == 0) n = 0;
+    __syncthreads();
     // the loop bound is uniform per workgroup, so every lane takes part in
-    // the ballots
+    // the ballots and barriers; an iteration adds at most 2 * kGenBlock keys
     for (uint64_t i0 = blockIdx.x * (uint64_t)kGenBlock; i0 < M; i0 += (uint64_t)gridDim.x * kGenBlock) {
         const uint64_t i = i0 + threadIdx.x;
         uint32_t u = 0, v = 0;
@@ -71,9 +95,12 @@ __global__ __launch_bounds__(kGenBlock) void k_gen_keys(Sampler s, uint64_t M, i
                 rev = sym && v >= lo && v < hi;
             }
         }
-        gen_append(fwd, ((unsigned long long)(u - lo) << vbits) | v, keys, count, cap);
-        gen_append(rev, ((unsigned long long)(v - lo) << vbits) | u, keys, count, cap);
+        gen_stage(fwd, ((unsigned long long)(u - lo) << vbits) | v, buf, &n);
+        gen_stage(rev, ((unsigned long long)(v - lo) << vbits) | u, buf, &n);
+        __syncthreads();
+        if (n > kGenStage - 2 * kGenBlock) gen_flush(buf, &n, &s_base, keys, count, cap);
     }
+    gen_flush(buf, &n, &s_base, keys, count, cap);
 }
 
 // first[r] / last[r]: start and end of range vertex r's run in the sorted
