@@ -436,7 +436,7 @@ __global__ __launch_bounds__(kPbBlock) void k_pb_gather(PrArgs a, const unsigned
 // segment.  The last pass over a block finishes its vertices (rank,
 // contribution, residual, dangling mass) as the pull does.  Destinations with
 // in-degree >= the heavy threshold keep the FP64 pull (k_pr_pull_heavy).
-constexpr int kTlMaxDstBits = 14;                          // LDS sums: 16384 FP32 (64 KB)
+constexpr int kTlMaxDstBits = 15;                          // LDS sums: up to 32768 FP32 (128 KB)
 constexpr uint32_t kTlMaxSeg = 8192;                       // build histograms in LDS
 constexpr int kTlBlock = 1024;                             // workgroup of the tiled pull
 
@@ -494,10 +494,16 @@ __global__ __launch_bounds__(kBlock) void k_tl_fill(const OffT* off, const uint3
 
 // Persistent: workgroup b takes destination blocks b, b + G, b + 2G, ...
 // (G = gridDim.x).
+// DB: the largest destination block (bits) its LDS sums hold; 2^14 blocks
+// (64 KB) leave room for two workgroups per CU, 2^15 (128 KB) for one.
+// Tiles are walked one by one (a flat walk across tile boundaries, each
+// word's segment found by a binary search in LDS, measured 30% slower: the
+// gathers of one pass then spread over several segments).
+template <int DB>
 __global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t* __restrict__ tiles,
                                                        const unsigned long long* __restrict__ toff, TileShape ts,
                                                        const uint32_t* heavy_bits) {
-    __shared__ float acc[1u << kTlMaxDstBits];
+    __shared__ float acc[1u << DB];
     __shared__ double s_r[kTlBlock / kWave], s_d[kTlBlock / kWave];
     const uint32_t G = gridDim.x;
     const uint32_t nb = blockIdx.x < ts.nd ? (ts.nd - blockIdx.x + G - 1) / G : 0;   // blocks of this workgroup
@@ -840,14 +846,18 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         set_last_error("propagation-blocked PageRank needs every edge resident, E < 2^32 and room for its slots");
         return ASC_ERR_UNSUPPORTED;
     }
-    // tile shape: 2^15-vertex segments (128 KB contribution slices) and
-    // 2^14-vertex blocks measured fastest on Erdos-Renyi 2^26 / 2^31 edges
-    // (20 iterations: 499 ms against 641-728 ms for 2^16-2^20-vertex segments
-    // and 821 ms for the pull); the graph's tuned shape or the caller's wins
+    // tile shape: 2^15-vertex segments (128 KB contribution slices) measured
+    // fastest on Erdos-Renyi 2^26 / 2^31 edges (20 iterations: 496 ms with
+    // 2^14-vertex blocks against 557-833 ms for 2^14 or 2^16-2^20-vertex
+    // segments, and 821 ms for the pull); 2^15-vertex blocks (one workgroup
+    // per CU, 128 KB of LDS sums, tiles twice as long) 468 ms, taken once
+    // there are at least four blocks per CU; the graph's tuned shape or the
+    // caller's wins
     const uint32_t sb = o->tile_src_bits > 0 ? (uint32_t)o->tile_src_bits
                         : g->pr_tile_src_bits > 0 ? (uint32_t)g->pr_tile_src_bits : 15;
+    const uint32_t db_auto = (V >> 15) >= 4 * (uint64_t)ctx->num_cus ? 15u : 14u;
     const uint32_t db = o->tile_dst_bits > 0 ? (uint32_t)o->tile_dst_bits
-                        : g->pr_tile_dst_bits > 0 ? (uint32_t)g->pr_tile_dst_bits : (uint32_t)kTlMaxDstBits;
+                        : g->pr_tile_dst_bits > 0 ? (uint32_t)g->pr_tile_dst_bits : db_auto;
     // AUTO: the tiled pull once the contributions outgrow an XCD's L2
     const bool want_tiled = o->method == ASC_PR_TILED ||
                             (o->method == ASC_PR_AUTO && g->static_vertices == V && V >= (uint64_t(1) << 20));
@@ -885,11 +895,17 @@ static int pr_normalised(asc_graph* g, const asc_pr_opts* o, float* rank_out, do
         a.skip_heavy = 1;
         if (tiled) {
             const TileShape ts{p->tl_sb, p->tl_db, p->tl_ns, p->tl_nd};
-            // two 1024-lane workgroups per CU (64 KB of LDS each)
-            const uint32_t tgrid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * 2, p->tl_nd);
-            hipLaunchKernelGGL(k_pr_tiled, dim3(tgrid), dim3(kTlBlock), 0, s, a, (const uint32_t*)p->tl_edges,
-                               (const unsigned long long*)p->tl_off, ts,
-                               p->n_heavy ? (const uint32_t*)p->tl_heavy : nullptr);
+            // 1024-lane workgroups: two per CU with blocks up to 2^14
+            // destinations (64 KB of LDS each), one with 2^15 (128 KB)
+            const bool big = p->tl_db > 14;
+            const uint32_t tgrid = (uint32_t)std::min<uint64_t>((uint64_t)ctx->num_cus * (big ? 1 : 2), p->tl_nd);
+            const uint32_t* hb = p->n_heavy ? (const uint32_t*)p->tl_heavy : nullptr;
+            if (big)
+                hipLaunchKernelGGL(k_pr_tiled<15>, dim3(tgrid), dim3(kTlBlock), 0, s, a,
+                                   (const uint32_t*)p->tl_edges, (const unsigned long long*)p->tl_off, ts, hb);
+            else
+                hipLaunchKernelGGL(k_pr_tiled<14>, dim3(tgrid), dim3(kTlBlock), 0, s, a,
+                                   (const uint32_t*)p->tl_edges, (const unsigned long long*)p->tl_off, ts, hb);
             ASC_HIP(hipGetLastError());
         } else if (blocked) {
             hipLaunchKernelGGL(k_pb_scatter, dim3(grid), dim3(kBlock), 0, s, p->pb_src, p->pb_pos, p->pb_edges,

@@ -85,6 +85,20 @@ def test_pr_blocked_and_pull_agree(gpu_ctx, case, method):
     assert np.allclose(res, ref_res, rtol=1e-3, atol=1e-7)
 
 
+@pytest.mark.parametrize("sb,db", [(15, 14), (15, 15), (12, 15), (13, 13), (17, 15)])
+def test_pr_tile_shapes(gpu_ctx, sb, db):
+    """Every source-segment / destination-block shape of the tiled pull meets
+    the L1 bound, 2^15-destination blocks (128 KB of LDS sums, one workgroup
+    per CU) included, on a directed graph with a ragged last block."""
+    V = 600000
+    off, dst, _ = G.random_graph(V, 1500000, 35, symmetric=False)
+    ioff, isrc, od = csc_of(off, dst, V)
+    ref, _ = O.pr_norm(O.CSR(ioff, isrc), od, 0.85, 20)
+    with gpu_ctx.from_arrays(ioff, isrc, out_degree=od) as g:
+        r, _, _ = g.pagerank(iterations=20, method=_abi.PR_TILED, tile_src_bits=sb, tile_dst_bits=db)
+    assert np.abs(r.astype(np.float64) - ref).sum() <= TOL_L1, (sb, db)
+
+
 def test_pr_er_2p22_normalised(gpu_ctx):
     """Erdos-Renyi with V = 2^22 and 2^24 undirected pairs (2^25 directed
     edges): the size where FP32 ranks and sums start to lose digits; every
