@@ -273,6 +273,27 @@ class Context:
         host_graph._owned = False
         return Graph(self, h)
 
+    def borrow_arrays(self, off, edges, kind=None, out_degree=None):
+        """Graph over the caller's numpy arrays without copying them
+        (asc_graph_borrow_csr); the Graph keeps references, and the arrays
+        must not change while it is open.  off: V+1 offsets (u64 borrowed,
+        u32 widened into a copy); edges: u32 [E] or u32 [E, 2] ({dst, w})."""
+        off = np.ascontiguousarray(off)
+        edges = np.ascontiguousarray(edges, dtype=np.uint32)
+        if kind is None:
+            kind = _abi.KIND_WCSR if edges.ndim == 2 else (_abi.KIND_CSC if out_degree is not None
+                                                            else _abi.KIND_CSR)
+        if out_degree is not None:
+            out_degree = np.ascontiguousarray(out_degree, dtype=np.uint32)
+        v = _view_from_arrays(off, edges, kind, out_degree)
+        v.num_vertices = len(off) - 1
+        v.has_sentinel = 1
+        h = C.c_void_p()
+        check(_abi.load().asc_graph_borrow_csr(self._h, C.byref(v), C.byref(h)), "asc_graph_borrow_csr")
+        g = Graph(self, h)
+        g._borrowed = (off, edges, out_degree)
+        return g
+
     def generate_graph(self, type=0, scale=10, num_vertices=0, num_samples=0, edge_factor=16, a=0.57, b=0.19,
                        c=0.19, seed=1, symmetrize=True, wmin=1, wmax=64, offset_bytes=8):
         """Weighted synthetic graph generated straight into a device graph

@@ -306,6 +306,7 @@ SIGNATURES = {
     "asc_host_csr_free": (None, [C.POINTER(HostCsr)]),
     "asc_host_csr_view": (None, [C.POINTER(HostCsr), C.POINTER(CsrView)]),
     "asc_graph_adopt": (C.c_int, [P, C.POINTER(HostCsr), C.c_int, PP]),
+    "asc_graph_borrow_csr": (C.c_int, [P, C.POINTER(CsrView), PP]),
     "asc_graph_generate": (C.c_int, [P, C.POINTER(GenParams), PP]),
     "asc_server_opts_init": (None, [C.POINTER(ServerOpts)]),
     "asc_server_create": (C.c_int, [P, C.POINTER(ServerOpts), PP]),

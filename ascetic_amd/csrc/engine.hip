@@ -372,6 +372,18 @@ int asc_graph_from_csr(asc_ctx* ctx, const asc_csr_view* view, asc_graph** out) 
     return finish_graph(ctx, g, out);
 }
 
+int asc_graph_borrow_csr(asc_ctx* ctx, const asc_csr_view* view, asc_graph** out) {
+    if (!ctx || !view || !out) return ASC_ERR_INVALID;
+    asc_graph* g = new asc_graph();
+    int r = csr_from_view(view, g->h, true);
+    if (r != ASC_OK) {
+        delete g;
+        return r;
+    }
+    g->off_bytes = g->h.offset_bytes;
+    return finish_graph(ctx, g, out);
+}
+
 int asc_graph_adopt(asc_ctx* ctx, asc_host_csr* h, int offset_bytes, asc_graph** out) {
     if (!ctx || !h || !out || !h->off || (h->num_edges && !h->edges)) return ASC_ERR_INVALID;
     if (offset_bytes != 4 && offset_bytes != 8) return ASC_ERR_INVALID;
@@ -428,8 +440,10 @@ int asc_graph_get_info(const asc_graph* g, asc_graph_info* info) {
     info->max_degree_vertex = g->max_degree_vertex;
     info->symmetric = g->symmetric;
     info->tail_pinned = g->static_edges < g->h.E && g->tail_pinned() ? 1 : 0;
-    info->host_bytes = 8 * (g->h.V + 1) + (g->h.edges.p ? (uint64_t)g->h.E * g->h.edge_bytes() : 0) +
-                       4ull * g->h.outdeg.n + 4ull * g->ptail.n + g->ptail_w.n + 4ull * g->zc_tail.n;
+    info->host_bytes = (g->h.off.borrowed ? 0 : 8 * (g->h.V + 1)) +
+                       (g->h.edges.p && !g->h.edges.borrowed ? (uint64_t)g->h.E * g->h.edge_bytes() : 0) +
+                       (g->h.outdeg.borrowed ? 0 : 4ull * g->h.outdeg.n) + 4ull * g->ptail.n + g->ptail_w.n +
+                       4ull * g->zc_tail.n;
     return ASC_OK;
 }
 

@@ -205,7 +205,7 @@ static int read_graph_as(FILE* f, const char* path, uint64_t size, int fmt, uint
     return validate_csr(out);
 }
 
-int csr_from_view(const asc_csr_view* v, HostCSR& out) {
+int csr_from_view(const asc_csr_view* v, HostCSR& out, bool borrow) {
     if (!v || v->num_vertices == 0 || (!v->off64 && !v->off32) || (v->num_edges && !v->edges)) {
         set_last_error("invalid csr view");
         return ASC_ERR_INVALID;
@@ -220,9 +220,6 @@ int csr_from_view(const asc_csr_view* v, HostCSR& out) {
     out.E = E;
     out.kind = v->kind;
     out.offset_bytes = v->off64 ? 8 : 4;
-    ASC_TRY(out.off.alloc(V + 1));
-    for (uint64_t i = 0; i < V; i++) out.off[i] = v->off64 ? v->off64[i] : v->off32[i];
-    out.off[V] = E;
     if (v->has_sentinel) {
         uint64_t s = v->off64 ? v->off64[V] : v->off32[V];
         if (s != E) {
@@ -230,11 +227,28 @@ int csr_from_view(const asc_csr_view* v, HostCSR& out) {
             return ASC_ERR_FORMAT;
         }
     }
-    ASC_TRY(out.edges.alloc(E * out.edge_words() + 1));
-    if (E) std::memcpy(out.edges.p, v->edges, E * out.edge_bytes());
+    // borrowed: the records and out-degrees (and u64 offsets with their
+    // sentinel) stay the caller's; narrower offsets are widened into a copy
+    if (borrow && v->off64 && v->has_sentinel) {
+        out.off.borrow(const_cast<uint64_t*>(v->off64), V + 1);
+    } else {
+        ASC_TRY(out.off.alloc(V + 1));
+        for (uint64_t i = 0; i < V; i++) out.off[i] = v->off64 ? v->off64[i] : v->off32[i];
+        out.off[V] = E;
+    }
+    if (borrow) {
+        if (E) out.edges.borrow(const_cast<uint32_t*>(v->edges), E * out.edge_words());
+    } else {
+        ASC_TRY(out.edges.alloc(E * out.edge_words() + 1));
+        if (E) std::memcpy(out.edges.p, v->edges, E * out.edge_bytes());
+    }
     if (v->kind == ASC_KIND_CSC) {
-        ASC_TRY(out.outdeg.alloc(V));
-        std::memcpy(out.outdeg.p, v->out_degree, 4 * V);
+        if (borrow) {
+            out.outdeg.borrow(const_cast<uint32_t*>(v->out_degree), V);
+        } else {
+            ASC_TRY(out.outdeg.alloc(V));
+            std::memcpy(out.outdeg.p, v->out_degree, 4 * V);
+        }
     }
     return validate_csr(out);
 }

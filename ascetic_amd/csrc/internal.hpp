@@ -54,6 +54,7 @@ struct HostBuf {
     T* p = nullptr;
     size_t n = 0;
     bool pinned = false;
+    bool borrowed = false;   // the caller's memory: never freed here
     HostBuf() = default;
     HostBuf(const HostBuf&) = delete;
     HostBuf& operator=(const HostBuf&) = delete;
@@ -61,8 +62,8 @@ struct HostBuf {
     HostBuf& operator=(HostBuf&& o) noexcept {
         if (this != &o) {
             release();
-            p = o.p; n = o.n; pinned = o.pinned;
-            o.p = nullptr; o.n = 0;
+            p = o.p; n = o.n; pinned = o.pinned; borrowed = o.borrowed;
+            o.p = nullptr; o.n = 0; o.borrowed = false;
         }
         return *this;
     }
@@ -96,12 +97,21 @@ struct HostBuf {
         if ((!max_bytes || count * sizeof(T) <= max_bytes) && alloc(count, true) == ASC_OK) return ASC_OK;
         return alloc(count, false);
     }
+    // Refers to `count` elements the caller owns and keeps alive.
+    void borrow(T* q, size_t count) {
+        release();
+        p = q;
+        n = count;
+        pinned = false;
+        borrowed = true;
+    }
     void release() {
-        if (p) {
+        if (p && !borrowed) {
             if (pinned) (void)hipHostFree(p); else std::free(p);
         }
         p = nullptr;
         n = 0;
+        borrowed = false;
     }
     T& operator[](size_t i) { return p[i]; }
     const T& operator[](size_t i) const { return p[i]; }
@@ -145,7 +155,7 @@ struct HostCSR {
 
 int read_graph_file(const char* path, int format, HostCSR& out);
 int write_graph_file(const char* path, int format, const asc_csr_view* v);
-int csr_from_view(const asc_csr_view* v, HostCSR& out);
+int csr_from_view(const asc_csr_view* v, HostCSR& out, bool borrow = false);
 int validate_csr(const HostCSR& h);
 int generate_graph(const asc_gen_params* p, HostCSR& out);
 int read_edge_list(const char* path, const asc_edge_list_opts* o, HostCSR& out);

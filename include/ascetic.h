@@ -229,6 +229,15 @@ typedef struct {
 int asc_graph_load(asc_ctx* ctx, const char* path, int format, asc_graph** out);
 /* Builds a graph from caller arrays (they are copied; the caller keeps them). */
 int asc_graph_from_csr(asc_ctx* ctx, const asc_csr_view* view, asc_graph** out);
+/* The same graph without copying the caller's records: the edge array, the
+ * out-degrees and u64 offsets with their sentinel are borrowed (narrower
+ * offsets are widened into a copy), and the caller keeps them alive and
+ * unchanged until asc_graph_destroy.  Host memory the library holds is then
+ * the offsets at most, plus the packed on-demand copy when a budget streams
+ * part of the graph.  Replaces the reference's host arrays filled by
+ * readDataFromFile (GraphMeta.cuh:151-171) for callers that already hold the
+ * graph. */
+int asc_graph_borrow_csr(asc_ctx* ctx, const asc_csr_view* view, asc_graph** out);
 int asc_graph_destroy(asc_graph* g);
 
 typedef struct {
