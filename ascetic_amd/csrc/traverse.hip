@@ -802,6 +802,8 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     // active edges per chunk -> which chunks are copied whole
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
     uint32_t n_dense = 0, n_sparse = 0, m = 0;
+    uint64_t sparse_edges = 0;           // active edges outside dense chunks
+    bool zc_sparse = false;
     std::vector<int> buf_of(n_ch, -1);   // part buffer holding each chunk whole (-1: none)
     if (n_ch) {
         ASC_HIP(hipMemsetAsync(t->d_ch_edges, 0, 8 * (size_t)n_ch, s));
@@ -825,9 +827,19 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             choose_dense_chunks(t, EB, buf_of);
         }
         for (uint32_t c = 0; c < n_ch; c++) {
-            if (t->h_ch_dense[c]) n_dense++;
-            else n_sparse += t->h_ch_count[c];
+            if (t->h_ch_dense[c]) {
+                n_dense++;
+            } else {
+                n_sparse += t->h_ch_count[c];
+                sparse_edges += t->h_ch_edges[c];
+            }
         }
+        // a sparse segment of at most half a part is read by the kernel
+        // straight from the page-locked tail: no list download, host prefix,
+        // gather or copy (C2's late rounds, 6M edges and below, where those
+        // cost more than the reads over the link)
+        zc_sparse = ctx->cfg.stream_mode == ASC_STREAM_GATHER && g->tail_pinned() && sparse_edges > 0 &&
+                    sparse_edges <= t->part_edges / 2;
         if (ctx->cfg.verbose > 2) {   // per-chunk active edges (D: copied whole)
             printf("round %u chunks:", rc.stream_rounds + 1);
             for (uint32_t c = 0; c < n_ch; c++)
@@ -853,7 +865,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_HIP(hipGetLastError());
         ASC_TRY(scan_degrees(t, list_deg, m, s));
         // the host gathers the sparse segment only: it alone comes back
-        if (n_sparse) {
+        if (n_sparse && !zc_sparse) {
             ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
             ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
         }
@@ -866,8 +878,10 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         // count, kept list and degrees (computed on the device) in one round
         // trip: the kept count is at most the pile size, so copy that many
         ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
-        ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
-        ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+        if (!zc_sparse) {
+            ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+            ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
+        }
         ASC_HIP(hipStreamSynchronize(s));
         m = t->h_piles[0].n_list;
         n_sparse = m;
@@ -879,10 +893,19 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     }
     const uint32_t* h_deg = t->h_list.p + nod;
     // u64 exclusive prefix of the sparse segment's degrees (the host gather's
-    // index; the same values as the device prefix), in parallel
-    const int nt = ctx->pool->size();
+    // index; the same values as the device prefix), in parallel; a segment
+    // read zero-copy needs only its total, the chunk counts' sum
+    const int nt = zc_sparse ? 1 : ctx->pool->size();
     std::vector<uint64_t> esum(nt + 1, 0);
     const uint64_t chunk = (n_sparse + nt - 1) / nt;
+    if (zc_sparse) {
+        // the device prefix's own total (the chunk counts agree with it; the
+        // kernel's range must be exact)
+        unsigned long long tot = 0;
+        ASC_HIP(hipMemcpyAsync(&tot, t->od_prefix + n_sparse, 8, hipMemcpyDeviceToHost, s));
+        ASC_HIP(hipStreamSynchronize(s));
+        esum[1] = tot;
+    } else {
     ctx->pool->run([&](int tt, int) {
         uint64_t b = std::min<uint64_t>(n_sparse, tt * chunk), e = std::min<uint64_t>(n_sparse, b + chunk), acc = 0;
         for (uint64_t i = b; i < e; i++) acc += h_deg[i];
@@ -897,6 +920,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             acc += h_deg[i];
         }
     });
+    }
     const uint64_t Ts = esum[nt];
     t->h_prefix[n_sparse] = Ts;
     // dense segments follow in chunk order; each holds its chunk's active
@@ -1039,6 +1063,21 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             rc.parts += nb;
             rc.streamed_edges += Ts + dense_edges;
             rc.dense_edges += dense_edges;
+            rc.overload_ms += now_ms() - t_begin;
+            return ASC_OK;
+        }
+        if (zc_sparse && Ts > 0) {
+            // the small sparse segment read from the mapped page-locked tail
+            ASC_TRY(map_tail());
+            p.tail_base = S;
+            p.lo = 0;
+            p.hi = Ts;
+            ASC_TRY(launch_part(true));
+            ASC_TRY(resident_step());
+            rc.parts += nb + 1;
+            rc.streamed_edges += Ts + dense_edges;
+            rc.dense_edges += dense_edges;
+            rc.fill_ms += fill_round;
             rc.overload_ms += now_ms() - t_begin;
             return ASC_OK;
         }
