@@ -867,10 +867,18 @@ def main():
             hr.free()
             hr = None
     if args.serve_scale > 0:
+        # steps replayed from hipGraphs ended the process with a host SIGSEGV
+        # inside the HIP runtime once under rocprofv3 --kernel-trace (cause not
+        # established, no device fault): under a profiler the steps are
+        # launched kernel by kernel instead
+        profiled = "ROCP_TOOL_LIBRARIES" in os.environ or any(k.startswith("ROCPROF_") for k in os.environ)
+        eager = args.serve_eager or profiled
+        if profiled and not args.serve_eager:
+            log("serving: profiler detected, engine steps launched kernel by kernel (no hipGraph replay)")
         # the serving section adopts (consumes) the resident graph when the scales match
         result["serving"] = serving_bench(A, O, args.serve_scale, args.seed, args.serve_queries,
-                                          [0.25, 1.0, 4.0, 8.0], cpu_threads, h=hr, step_graphs=not args.serve_eager)
-        result["serving"]["step_graphs"] = not args.serve_eager
+                                          [0.25, 1.0, 4.0, 8.0], cpu_threads, h=hr, step_graphs=not eager)
+        result["serving"]["step_graphs"] = not eager
         hr = None
     if args.c4_scale > 0:
         result["c4_shape"] = c4_shape_section(A, _abi, O, args, cpu_threads)
