@@ -983,9 +983,12 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     // and each step takes its own level, so a resident vertex improved again
     // after its step is queued again (stamps are per level); BFS keeps one
     // level per round because its values are levels.
-    const bool overlap = ALGO != ALG_BFS && ctx->cfg.stream_overlap != 0;
+    const bool overlap = ALGO != ALG_BFS && ctx->cfg.stream_overlap > 0;
+    const int every = std::max(1, ctx->cfg.stream_overlap);
+    int since = 0;   // parts or chunks launched since the last resident step
     auto resident_step = [&]() -> int {
-        if (!overlap) return ASC_OK;
+        if (!overlap || ++since < every) return ASC_OK;
+        since = 0;
         level++;
         a.q_in = t->q[level & 1];
         a.q_out = t->q[(level + 1) & 1];

@@ -156,10 +156,10 @@ typedef struct {
                                     pinned tail instead of gathered; 0 = chosen
                                     per round (at least half active, then by a
                                     link/gather balance), > 1 = never        */
-    int stream_overlap;          /* SSSP/CC: 1 (default) = expand the resident
-                                    frontier after every streamed part or
-                                    chunk, while the next one is in flight;
-                                    0 = after the whole round                */
+    int stream_overlap;          /* SSSP/CC: n >= 1 = expand the resident
+                                    frontier after every n-th streamed part or
+                                    chunk, while the next one is in flight
+                                    (default 1); 0 = after the whole round  */
     int bfs_persistent;          /* direction-optimizing BFS: 1 = the whole
                                     traversal in one persistent cooperative
                                     kernel that decides each level's direction

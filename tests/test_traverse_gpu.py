@@ -143,20 +143,21 @@ def test_dense_chunk_copies(name, mk, thr):
         assert dense > 0
 
 
+@pytest.mark.parametrize("every", [1, 3])
 @pytest.mark.parametrize("pack", [1, 2])
 @pytest.mark.parametrize("thr", [1e-9, 2.0])
 @pytest.mark.parametrize("name,mk", CASES[3:])
-def test_stream_overlap_exact(name, mk, thr, pack):
-    """Resident steps between streamed chunks / parts (SSSP, CC) give the
-    oracle's values, with whole-chunk copies and with gathering only, in the
-    4-byte and the split 5-byte stream forms."""
-    say("overlap", name, "threshold", thr, "pack", pack)
+def test_stream_overlap_exact(name, mk, thr, pack, every):
+    """Resident steps between streamed chunks / parts (SSSP, CC; after every
+    part or every third) give the oracle's values, with whole-chunk copies and
+    with gathering only, in the 4-byte and the split 5-byte stream forms."""
+    say("overlap", name, "threshold", thr, "pack", pack, "every", every)
     off, dst, _ = mk()
     w = weights_for(off, 8)
     csr = O.CSR(off, dst, w)
     E = int(off[-1])
     with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 2048, dense_threshold=thr,
-                   stream_overlap=True, stream_pack=pack) as ctx:
+                   stream_overlap=every, stream_pack=pack) as ctx:
         with make_graph(ctx, off, dst, w) as g:
             for s in (0, int(np.argmax(np.diff(off))), len(off) - 2):
                 d, st = g.sssp(s)
