@@ -107,7 +107,8 @@ class KernelStats(C.Structure):
     ]
 
 
-KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull", "bottom_up", "union_find", "bfs_persistent")
+KERNEL_NAMES = ("expand_local", "expand_heavy", "expand_part", "pr_pull", "bottom_up", "union_find", "bfs_persistent",
+                "expand_part_host")
 PATH_NAMES = ("frontier", "direction_opt", "union_find")
 
 
@@ -134,7 +135,7 @@ class Stats(C.Structure):
         ("pr_residual", C.c_double),
         ("host_cores", C.c_int),
         ("path", C.c_int),
-        ("kern", KernelStats * 7),
+        ("kern", KernelStats * 8),
         ("reused_edges", C.c_uint64),
     ]
 

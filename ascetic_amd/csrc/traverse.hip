@@ -47,11 +47,15 @@ namespace asc {
 // Edge-balanced relax over edges [lo, hi) of the concatenated adjacency of
 // the streamed list (list[k] owns [prefix[k], prefix[k+1]), no zero-degree
 // entries).  SSSP/CC expand each entry with its value when its tile starts,
-// BFS with the level.  ZC = false: edges
-// come from the device part buffer holding [lo, hi).  ZC = true: from mapped
-// pinned host memory (tail = edges from static_edges on).
-template <int ALGO, class OffT, class EdgeT, bool ZC, bool PK>
+// BFS with the level.  SRC = PART_GATHERED: edges come from the device part
+// buffer holding [lo, hi) of the list's concatenation.  Otherwise they are
+// addressed by offset from p.tail_base: SRC = PART_CHUNK, a dense chunk copied
+// whole into a device part buffer; SRC = PART_HOST, mapped page-locked host
+// memory (the tail from static_edges on).  PART_HOST launches count apart.
+enum { PART_GATHERED = 0, PART_CHUNK = 1, PART_HOST = 2 };
+template <int ALGO, class OffT, class EdgeT, int SRC, bool PK>
 __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a, uint32_t level, PartArgs<EdgeT> p) {
+    constexpr bool ZC = SRC != PART_GATHERED;
     __shared__ unsigned long long s_p[kTile];
     __shared__ uint32_t s_sv[kTile];
     __shared__ unsigned long long s_src[ZC ? kTile : 1];
@@ -169,8 +173,8 @@ __global__ __launch_bounds__(kBlock) void k_expand_part(TravArgs<OffT, EdgeT> a,
     }
     pq.flush(a, nxt);
     if (tid == 0 && my_edges) {
-        atomicAdd(&a.stats->edges_part, my_edges);
-        atomicAdd(&a.stats->verts_part, my_verts);
+        atomicAdd(SRC == PART_HOST ? &a.stats->edges_part_host : &a.stats->edges_part, my_edges);
+        atomicAdd(SRC == PART_HOST ? &a.stats->verts_part_host : &a.stats->verts_part, my_verts);
     }
 }
 
@@ -960,22 +964,24 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         }
         return ASC_OK;
     };
-    auto launch_part = [&](bool zc) {
+    auto launch_part = [&](int src) {
         const int pgrid = (int)std::max<uint64_t>(1, std::min<uint64_t>(grid, (p.hi - p.lo + kTile - 1) / kTile));
-        return timed_launch(t, prof, KERN_PART, s, [&] {
-            const int grid = pgrid;
-            if (zc && pk)
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, true>), dim3(grid), dim3(kBlock), 0, s, a,
-                                   level, p);
-            else if (zc)
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, true, false>), dim3(grid), dim3(kBlock), 0, s, a,
-                                   level, p);
+        return timed_launch(t, prof, src == PART_HOST ? KERN_PART_HOST : KERN_PART, s, [&] {
+            const dim3 gd(pgrid), bd(kBlock);
+            if (src == PART_HOST && pk)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_HOST, true>), gd, bd, 0, s, a, level, p);
+            else if (src == PART_HOST)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_HOST, false>), gd, bd, 0, s, a, level, p);
+            else if (src == PART_CHUNK && pk)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_CHUNK, true>), gd, bd, 0, s, a, level, p);
+            else if (src == PART_CHUNK)
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_CHUNK, false>), gd, bd, 0, s, a, level, p);
             else if (pk)
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, true>), dim3(grid), dim3(kBlock), 0, s,
-                                   a, level, p);
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_GATHERED, true>), gd, bd, 0, s, a, level,
+                                   p);
             else
-                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, false, false>), dim3(grid), dim3(kBlock), 0, s,
-                                   a, level, p);
+                hipLaunchKernelGGL((k_expand_part<ALGO, OffT, EdgeT, PART_GATHERED, false>), gd, bd, 0, s, a, level,
+                                   p);
         });
     };
     // SSSP/CC with stream_overlap: after each part or chunk, one resident step
@@ -1006,7 +1012,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
         ASC_TRY(map_tail());
         p.lo = 0;
         p.hi = T;
-        ASC_TRY(launch_part(true));
+        ASC_TRY(launch_part(PART_HOST));
         rc.parts += 1;
         rc.streamed_edges += T;
     } else {
@@ -1049,7 +1055,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             p.tail_base = e0;
             p.lo = t->h_ch_lo[c];
             p.hi = t->h_ch_lo[c] + t->h_ch_edges[c];
-            if (p.hi > p.lo) ASC_TRY(launch_part(true));
+            if (p.hi > p.lo) ASC_TRY(launch_part(PART_CHUNK));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
         }
@@ -1060,7 +1066,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                 p.tail_base = S;
                 p.lo = 0;
                 p.hi = Ts;
-                ASC_TRY(launch_part(true));
+                ASC_TRY(launch_part(PART_HOST));
                 nb++;
             }
             rc.parts += nb;
@@ -1075,7 +1081,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             p.tail_base = S;
             p.lo = 0;
             p.hi = Ts;
-            ASC_TRY(launch_part(true));
+            ASC_TRY(launch_part(PART_HOST));
             ASC_TRY(resident_step());
             rc.parts += nb + 1;
             rc.streamed_edges += Ts + dense_edges;
@@ -1109,7 +1115,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             p.tail_base = S;
             p.lo = lo;
             p.hi = hi;
-            ASC_TRY(launch_part(false));
+            ASC_TRY(launch_part(PART_GATHERED));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
         }
@@ -1322,6 +1328,8 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
     kern[KERN_HEAVY].edges = ds.edges_heavy;
     kern[KERN_PART].edges = ds.edges_part;
     kern[KERN_PART].vertices = ds.verts_part;
+    kern[KERN_PART_HOST].edges = ds.edges_part_host;
+    kern[KERN_PART_HOST].vertices = ds.verts_part_host;
     kern[KERN_BU].edges = ds.edges_bu;
     kern[KERN_BU].vertices = ds.verts_bu;
     kern[KERN_UF].edges = ds.edges_uf;
@@ -1333,14 +1341,14 @@ int finish_traversal(asc_graph* g, int algo, int path, uint32_t* out, asc_stats*
         st->total_ms = total_ms;
         st->pre_ms = rc.pre_ms;
         st->static_ms = static_ms;
-        st->kernel_ms = static_ms + kern[KERN_PART].ms;
+        st->kernel_ms = static_ms + kern[KERN_PART].ms + kern[KERN_PART_HOST].ms;
         st->overload_ms = rc.overload_ms;
         st->fill_ms = rc.fill_ms;
         st->iterations = iterations;
         st->parts = rc.parts;
         st->node_sum = rc.node_sum;
         st->static_edges_traversed = ds.edges_local + ds.edges_heavy + ds.edges_bu + ds.edges_uf;
-        st->edges_traversed = st->static_edges_traversed + ds.edges_part;
+        st->edges_traversed = st->static_edges_traversed + ds.edges_part + ds.edges_part_host;
         st->streamed_edges = rc.streamed_edges;
         st->streamed_bytes = rc.streamed_edges * g->stream_record_bytes();
         st->dense_edges = rc.dense_edges;

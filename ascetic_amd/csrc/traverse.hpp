@@ -42,8 +42,10 @@ struct DevStats {
     unsigned long long edges_local;   // k_expand_local
     unsigned long long verts_local;
     unsigned long long edges_heavy;   // k_expand_heavy
-    unsigned long long edges_part;    // k_expand_part
+    unsigned long long edges_part;    // k_expand_part reading a device part buffer
     unsigned long long verts_part;
+    unsigned long long edges_part_host;   // k_expand_part reading mapped host memory
+    unsigned long long verts_part_host;
     unsigned long long edges_bu;      // k_bfs_bottom_up: edge records read
     unsigned long long verts_bu;      //   unvisited vertices scanned
     unsigned long long edges_uf;      // union-find link passes: edge records read
@@ -51,7 +53,7 @@ struct DevStats {
 
 enum {
     KERN_LOCAL = 0, KERN_HEAVY = 1, KERN_PART = 2, KERN_PR = 3, KERN_BU = 4, KERN_UF = 5, KERN_PERSIST = 6,
-    KERN_COUNT = 7
+    KERN_PART_HOST = 7, KERN_COUNT = 8
 };
 
 // asc_stats.path

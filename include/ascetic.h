@@ -314,10 +314,12 @@ typedef struct {
     /* Per kernel kind; durations filled when asc_config.profile = 1, edge
      * and vertex counts always:
      * [0] resident frontier expansion, [1] grid-cooperative heavy vertices,
-     * [2] on-demand parts, [3] PageRank pull, [4] BFS bottom-up steps,
-     * [5] union-find link passes, [6] the persistent BFS kernel (whole
-     * traversal; its edges and vertices are counted under [0], [1], [4]). */
-    asc_kernel_stats kern[7];
+     * [2] on-demand parts read from a device part buffer, [3] PageRank
+     * pull, [4] BFS bottom-up steps, [5] union-find link passes, [6] the
+     * persistent BFS kernel (whole traversal; its edges and vertices are
+     * counted under [0], [1], [4]), [7] on-demand parts read straight from
+     * mapped host memory (zero-copy and hybrid streaming, small rounds). */
+    asc_kernel_stats kern[8];
     uint64_t reused_edges;       /* on-demand edges read again from a chunk an
                                     earlier round of the run left in a part
                                     buffer (no transfer; not in streamed_edges) */

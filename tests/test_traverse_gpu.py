@@ -91,6 +91,12 @@ def test_streaming_matches_resident(name, mk, mode):
             lab, st = g.cc()
             assert np.array_equal(lab, O.cc_minreach(csr))
             assert st["streamed_edges"] > 0
+            # on-demand edges are counted under the device-buffer part kernel
+            # or, when read from mapped host memory, under expand_part_host
+            kp, kh = st["kernels"]["expand_part"], st["kernels"]["expand_part_host"]
+            assert kp["edges"] + kh["edges"] == st["edges_traversed"] - st["static_edges_traversed"] > 0
+            if mode == _abi.STREAM_ZEROCOPY:
+                assert kp["edges"] == 0 and kh["edges"] > 0
     # stream_pack 1: 4-byte words; 2: u32 ids plus weight bytes; 0: 8-byte {dst, w} records
     for pack, rec in ((1, 4), (2, 5), (0, 8)):
         with A.Context(edge_budget_bytes=8 * (E // 3), part_bytes=8 * 4096, stream_mode=mode,
