@@ -185,12 +185,11 @@ int hardware_threads();
 // Process-wide pool for host-side preprocessing (generation, loading).
 ThreadPool& host_pool();
 
-// Copies edges [lo, hi) of the concatenated adjacency of list[0..m)
-// (list[k] owns [prefix[k], prefix[k+1])) into dst.  `edges` holds records
-// of EB bytes for edge indices starting at `base` (edge i at edges[i - base]).
-void gather_edges(ThreadPool& pool, const char* edges, uint64_t base, const uint64_t* off, size_t EB,
-                  const uint32_t* list, const unsigned long long* prefix, uint64_t m, uint64_t lo, uint64_t hi,
-                  char* dst);
+// Copies edges [lo, hi) of the concatenation of m pieces into dst: piece k
+// owns [prefix[k], prefix[k+1]) of the concatenation and starts at record
+// src[k] of `edges` (records of EB bytes).
+void gather_edges(ThreadPool& pool, const char* edges, size_t EB, const unsigned long long* src,
+                  const unsigned long long* prefix, uint64_t m, uint64_t lo, uint64_t hi, char* dst);
 
 }  // namespace asc
 

@@ -561,7 +561,8 @@ int trav_ensure(asc_graph* g) {
         ASC_HIP(hipMalloc(&t->od_list, 8 * nod));
         ASC_HIP(hipMalloc(&t->od_prefix, 8 * (nod + 1)));
         ASC_HIP(hipMalloc(&t->scan_tmp, 8 * (nod / kScanTile + 2)));
-        ASC_TRY(t->h_list.alloc(2 * nod, true));
+        ASC_TRY(t->h_src.alloc(nod, true));
+        ASC_HIP(hipHostGetDevicePointer((void**)&t->d_hsrc, t->h_src.p, 0));
         ASC_TRY(t->h_prefix.alloc(nod + 1, true));
         t->part_edges = g->part_edges;
         const int nb_cfg = g->ctx->cfg.part_buffers;
@@ -640,17 +641,16 @@ static uint32_t sssp_delta(asc_graph* g) {
 // Copies the adjacency of list entries overlapping [lo, hi) of the
 // concatenated on-demand edge list into `dst` (the reference's
 // fillEdgeArrByMultiThread, GraphMeta.cuh:469-514, cut to one part).
-static void gather_part(asc_graph* g, const uint32_t* list, const unsigned long long* prefix, uint64_t m,
+// src[k]: first record of the k-th listed vertex in the array gathered from
+// (the packed tail, indexed from static_edges on, or the host records).
+static void gather_part(asc_graph* g, const unsigned long long* src, const unsigned long long* prefix, uint64_t m,
                         uint64_t lo, uint64_t hi, char* dst, char* dst_w) {
     if (g->packed) {
-        gather_edges(*g->ctx->pool, (const char*)g->ptail.p, g->static_edges, g->h.off.p, 4, list, prefix, m, lo, hi,
-                     dst);
+        gather_edges(*g->ctx->pool, (const char*)g->ptail.p, 4, src, prefix, m, lo, hi, dst);
         if (g->pk_split)   // the weight bytes, at the same positions of their own array
-            gather_edges(*g->ctx->pool, (const char*)g->ptail_w.p, g->static_edges, g->h.off.p, 1, list, prefix, m,
-                         lo, hi, dst_w);
+            gather_edges(*g->ctx->pool, (const char*)g->ptail_w.p, 1, src, prefix, m, lo, hi, dst_w);
     } else
-        gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, 0, g->h.off.p, g->h.edge_bytes(), list, prefix, m, lo,
-                     hi, dst);
+        gather_edges(*g->ctx->pool, (const char*)g->h.edges.p, g->h.edge_bytes(), src, prefix, m, lo, hi, dst);
 }
 
 // Chooses the chunks copied whole: every chunk with at least half of its
@@ -689,6 +689,17 @@ static void choose_dense_chunks(TravState* t, size_t EB, const std::vector<int>&
 // Copies `cnt` (at most one part) streamed records into device part buffer b:
 // whole records from `src`, or in the split form the ids from `src` and the
 // weight bytes from `src_w`, placed behind the buffer's part_edges ids.
+// Orders stream `s` after `ev`, unless the host already sees `ev` complete
+// (a part buffer whose last reader or copy ended long ago): the wait would
+// only add a dependency between the copy and compute queues.
+static int wait_event(hipStream_t s, hipEvent_t ev) {
+    const hipError_t q = hipEventQuery(ev);
+    if (q == hipSuccess) return ASC_OK;
+    if (q != hipErrorNotReady) ASC_HIP(q);
+    ASC_HIP(hipStreamWaitEvent(s, ev, 0));
+    return ASC_OK;
+}
+
 static int copy_to_part(asc_graph* g, TravState* t, int b, const char* src, const char* src_w, uint64_t cnt,
                         hipStream_t s) {
     if (cnt > t->part_edges) {
@@ -768,12 +779,21 @@ static int prefetch_chunks(asc_graph* g, TravState* t) {
     for (int b = 0; b < t->n_bufs && (uint32_t)b < n_ch; b++) {
         const uint64_t e0 = t->ch_e[b], e1 = t->ch_e[b + 1];
         if (e1 - e0 > t->part_edges) break;
-        ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));
+        ASC_TRY(wait_event(ctx->s_copy, t->ev_kern[b]));
         ASC_TRY(copy_chunk_to_part(g, t, b, e0, e1, ctx->s_copy, nullptr));
         ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
         t->buf_chunk[b] = b;
     }
     return ASC_OK;
+}
+
+// First record of each listed vertex's edges in the array the host gathers
+// from (offset minus `base`), written to mapped page-locked host memory.
+template <class OffT>
+__global__ __launch_bounds__(kBlock) void k_list_src(const OffT* off, const uint32_t* list, uint32_t n,
+                                                      unsigned long long base, unsigned long long* out) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+        out[i] = (unsigned long long)off[list[i]] - base;
 }
 
 // od_prefix[0..n] = exclusive u64 prefix of the n degrees (device only).
@@ -802,6 +822,18 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
     const size_t EB = g->stream_record_bytes();
     const uint64_t S = g->static_edges;
     const int pile_grid = (int)std::max<uint32_t>(1, std::min<uint32_t>((npile + kBlock - 1) / kBlock, 4096));
+    // the host gather's index of the first n list entries (the sparse
+    // segment): each vertex's first record in the array gathered from,
+    // written by the device into mapped host memory, and the device's u64
+    // prefix of their degrees
+    auto gather_index = [&](uint32_t n) -> int {
+        const int ig = (int)std::max<uint32_t>(1, std::min<uint32_t>((n + kBlock - 1) / kBlock, 4096));
+        hipLaunchKernelGGL((k_list_src<OffT>), dim3(ig), dim3(kBlock), 0, s, a.off, (const uint32_t*)t->od_list, n,
+                           (unsigned long long)(pk ? S : 0), t->d_hsrc);
+        ASC_HIP(hipGetLastError());
+        ASC_HIP(hipMemcpyAsync(t->h_prefix.p, t->od_prefix, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+        return ASC_OK;
+    };
 
     // active edges per chunk -> which chunks are copied whole
     const uint32_t n_ch = t->ch_v.empty() ? 0 : (uint32_t)t->ch_v.size() - 1;
@@ -868,64 +900,40 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                            (const uint8_t*)t->d_ch_dense, t->d_cursor);
         ASC_HIP(hipGetLastError());
         ASC_TRY(scan_degrees(t, list_deg, m, s));
-        // the host gathers the sparse segment only: it alone comes back
-        if (n_sparse && !zc_sparse) {
-            ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
-            ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)n_sparse, hipMemcpyDeviceToHost, s));
-        }
+        // the host gathers the sparse segment only: its index alone comes back
+        if (n_sparse && !zc_sparse && ctx->cfg.stream_mode == ASC_STREAM_GATHER) ASC_TRY(gather_index(n_sparse));
         ASC_HIP(hipStreamSynchronize(s));
     } else {
         hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_list, 0u);
         hipLaunchKernelGGL((k_filter_pile<ALGO, OffT>), dim3(pile_grid), dim3(kBlock), 0, s, a.off, pile, npile,
                            a.val, a.done, t->od_list, list_deg, &t->piles->n_list);
         ASC_HIP(hipGetLastError());
-        // count, kept list and degrees (computed on the device) in one round
-        // trip: the kept count is at most the pile size, so copy that many
         ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
-        if (!zc_sparse) {
-            ASC_HIP(hipMemcpyAsync(t->h_list.p, t->od_list, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
-            ASC_HIP(hipMemcpyAsync(t->h_list.p + nod, list_deg, 4 * (size_t)npile, hipMemcpyDeviceToHost, s));
-        }
         ASC_HIP(hipStreamSynchronize(s));
         m = t->h_piles[0].n_list;
         n_sparse = m;
-        if (m) ASC_TRY(scan_degrees(t, list_deg, m, s));
+        if (m) {
+            ASC_TRY(scan_degrees(t, list_deg, m, s));
+            if (ctx->cfg.stream_mode == ASC_STREAM_GATHER) ASC_TRY(gather_index(m));
+            ASC_HIP(hipStreamSynchronize(s));
+        }
     }
     if (m == 0) {
         rc.pre_ms += now_ms() - t_begin;
         return ASC_OK;
     }
-    const uint32_t* h_deg = t->h_list.p + nod;
-    // u64 exclusive prefix of the sparse segment's degrees (the host gather's
-    // index; the same values as the device prefix), in parallel; a segment
-    // read zero-copy needs only its total, the chunk counts' sum
-    const int nt = zc_sparse ? 1 : ctx->pool->size();
-    std::vector<uint64_t> esum(nt + 1, 0);
-    const uint64_t chunk = (n_sparse + nt - 1) / nt;
-    if (zc_sparse) {
-        // the device prefix's own total (the chunk counts agree with it; the
-        // kernel's range must be exact)
+    // edges of the sparse segment: the device prefix's entry at its end (the
+    // chunk counts agree with it; the kernel's range must be exact), already
+    // on the host with the gather index
+    uint64_t Ts = 0;
+    if (n_sparse && !zc_sparse && ctx->cfg.stream_mode == ASC_STREAM_GATHER) {
+        Ts = t->h_prefix[n_sparse];
+    } else if (n_sparse) {   // read by the kernel from mapped memory: the total alone
         unsigned long long tot = 0;
         ASC_HIP(hipMemcpyAsync(&tot, t->od_prefix + n_sparse, 8, hipMemcpyDeviceToHost, s));
         ASC_HIP(hipStreamSynchronize(s));
-        esum[1] = tot;
-    } else {
-    ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(n_sparse, tt * chunk), e = std::min<uint64_t>(n_sparse, b + chunk), acc = 0;
-        for (uint64_t i = b; i < e; i++) acc += h_deg[i];
-        esum[tt + 1] = acc;
-    });
-    for (int i = 0; i < nt; i++) esum[i + 1] += esum[i];
-    ctx->pool->run([&](int tt, int) {
-        uint64_t b = std::min<uint64_t>(n_sparse, tt * chunk), e = std::min<uint64_t>(n_sparse, b + chunk);
-        uint64_t acc = esum[tt];
-        for (uint64_t i = b; i < e; i++) {
-            t->h_prefix[i] = acc;
-            acc += h_deg[i];
-        }
-    });
+        Ts = tot;
     }
-    const uint64_t Ts = esum[nt];
     t->h_prefix[n_sparse] = Ts;
     // dense segments follow in chunk order; each holds its chunk's active
     // edges, so their bounds in the prefix follow from the chunk counts
@@ -1039,7 +1047,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                 // the chunks read in place were all launched already, so any
                 // buffer may be overwritten once its last reader is done
                 b = (int)(nb++ % t->n_bufs);
-                ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));   // last reader of part b done
+                ASC_TRY(wait_event(ctx->s_copy, t->ev_kern[b]));   // last reader of part b done
                 ASC_TRY(copy_chunk_to_part(g, t, b, e0, e1, ctx->s_copy, &fill_round));
                 ASC_HIP(hipEventRecord(t->ev_copy[b], ctx->s_copy));
                 dense_edges += e1 - e0;
@@ -1048,7 +1056,7 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
                 buf_of[c] = b;
             }
             t->buf_counted[b] = true;
-            ASC_HIP(hipStreamWaitEvent(s, t->ev_copy[b], 0));
+            ASC_TRY(wait_event(s, t->ev_copy[b]));
             p.zc = (const EdgeT*)t->part[b];
             p.zc32 = (const uint32_t*)t->part[b];
             p.zc_w8 = (const uint8_t*)t->part[b] + 4 * PE;
@@ -1058,6 +1066,9 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             if (p.hi > p.lo) ASC_TRY(launch_part(PART_CHUNK));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
+            if (ctx->cfg.verbose > 2)
+                printf("  chunk %u (buffer %d%s) enqueued at %.3f ms\n", c, b, oi < n_cached ? ", held" : "",
+                       now_ms() - rc.t_start);
         }
         if (ctx->cfg.stream_mode == ASC_STREAM_HYBRID) {
             // the rest read by the kernel from the mapped pinned tail
@@ -1099,11 +1110,11 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             ASC_HIP(hipEventSynchronize(t->ev_stage[sb]));  // staging sb drained
             const double tg = now_ms();
             char* stage_w = t->stage[sb].p + 4 * PE;   // split form: weight bytes behind the ids
-            gather_part(g, t->h_list.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p, stage_w);
+            gather_part(g, t->h_src.p, t->h_prefix.p, n_sparse, lo, hi, t->stage[sb].p, stage_w);
             const double dt = now_ms() - tg;
             fill_round += dt;
             gather_round += dt;
-            ASC_HIP(hipStreamWaitEvent(ctx->s_copy, t->ev_kern[b], 0));  // device part b free
+            ASC_TRY(wait_event(ctx->s_copy, t->ev_kern[b]));  // device part b free
             t->buf_chunk[b] = -1;                                       // now holds gathered records
             ASC_TRY(copy_to_part(g, t, b, t->stage[sb].p, stage_w, hi - lo, ctx->s_copy));
             ASC_HIP(hipEventRecord(t->ev_stage[sb], ctx->s_copy));
@@ -1118,6 +1129,9 @@ static int stream_pile(asc_graph* g, TravArgs<OffT, EdgeT>& a, uint32_t& level, 
             ASC_TRY(launch_part(PART_GATHERED));
             ASC_HIP(hipEventRecord(t->ev_kern[b], s));
             ASC_TRY(resident_step());
+            if (ctx->cfg.verbose > 2)
+                printf("  part %llu (buffer %d) gathered in %.3f ms, enqueued at %.3f ms\n", (unsigned long long)q, b,
+                       dt, now_ms() - rc.t_start);
         }
         rc.fill_ms += fill_round;
         rc.parts += nb;

@@ -154,8 +154,11 @@ struct TravState {
     HostBuf<Slot> h_slot;
     HostBuf<Piles> h_piles;
     HostBuf<DevStats> h_stats;
-    HostBuf<uint32_t> h_od, h_list;
-    HostBuf<unsigned long long> h_prefix;
+    // the host gather's index of a round's sparse segment: first record of
+    // each listed vertex in the gathered array (written by the device through
+    // d_hsrc, mapped) and the u64 exclusive prefix of their degrees
+    HostBuf<unsigned long long> h_src, h_prefix;
+    unsigned long long* d_hsrc = nullptr;
     HostBuf<char> stage[2];
     uint64_t stage_seq = 0;         // uses of the two staging buffers (gather and staged chunk copies)
     // dense on-demand chunks: vertex ranges [ch_v[i], ch_v[i+1]) whose edges
