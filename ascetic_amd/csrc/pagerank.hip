@@ -498,7 +498,11 @@ __global__ __launch_bounds__(kBlock) void k_tl_fill(const OffT* off, const uint3
 // (64 KB) leave room for two workgroups per CU, 2^15 (128 KB) for one.
 // Tiles are walked one by one (a flat walk across tile boundaries, each
 // word's segment found by a binary search in LDS, measured 30% slower: the
-// gathers of one pass then spread over several segments).
+// gathers of one pass then spread over several segments).  A tile is
+// usually one pass (ER 2^26: ~512 words for 1024 lanes), so a pass is two
+// dependent memory round trips (its words, then their gathers): the words of
+// the next tile's first pass are loaded before the current tile's gathers,
+// which leaves one round trip per tile.
 template <int DB>
 __global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t* __restrict__ tiles,
                                                        const unsigned long long* __restrict__ toff, TileShape ts,
@@ -515,24 +519,47 @@ __global__ __launch_bounds__(kTlBlock) void k_pr_tiled(PrArgs a, const uint32_t*
         for (uint32_t i = threadIdx.x; i < dsz; i += kTlBlock) acc[i] = 0.0f;
         __syncthreads();
         const unsigned long long* t = toff + d * ts.ns;
-        unsigned long long k0 = t[0];
+        unsigned long long k0 = t[0], k1 = t[1];
+        // first pass of a tile: four words per lane, lanes past its end load nothing
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const unsigned long long k = k0 + threadIdx.x + j * kTlBlock;
+            w[j] = k < k1 ? tiles[k] : 0u;
+        }
         for (uint32_t s = 0; s < ts.ns; s++) {
-            const unsigned long long k1 = t[s + 1];
+            const bool more = s + 1 < ts.ns;
+            const unsigned long long k2 = more ? t[s + 2] : k1;
+            uint32_t wn[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const unsigned long long k = k1 + threadIdx.x + j * kTlBlock;
+                wn[j] = k < k2 ? tiles[k] : 0u;
+            }
             const float* cseg = a.contrib_in + ((uint64_t)s << ts.sb);
-            // four words per lane per pass: the words, then their gathers,
-            // then the LDS adds (lanes past the tile's end re-read its first word)
-            for (unsigned long long k = k0 + threadIdx.x; k < k1; k += 4 * kTlBlock) {
-                uint32_t w[4];
+            float c[4];
 #pragma unroll
-                for (int j = 0; j < 4; j++) w[j] = tiles[k + j * kTlBlock < k1 ? k + j * kTlBlock : k0];
-                float c[4];
+            for (int j = 0; j < 4; j++) c[j] = k0 + threadIdx.x + j * kTlBlock < k1 ? cseg[w[j] & smask] : 0.0f;
 #pragma unroll
-                for (int j = 0; j < 4; j++) c[j] = cseg[w[j] & smask];
+            for (int j = 0; j < 4; j++)
+                if (k0 + threadIdx.x + j * kTlBlock < k1) atomicAdd(&acc[w[j] >> ts.sb], c[j]);
+            // further passes of a tile longer than one: the words, then their
+            // gathers, then the LDS adds (lanes past the end re-read its first word)
+            for (unsigned long long k = k0 + threadIdx.x + 4 * kTlBlock; k < k1; k += 4 * kTlBlock) {
+                uint32_t x[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) x[j] = tiles[k + j * kTlBlock < k1 ? k + j * kTlBlock : k0];
+                float cx[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) cx[j] = cseg[x[j] & smask];
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    if (k + j * kTlBlock < k1) atomicAdd(&acc[w[j] >> ts.sb], c[j]);
+                    if (k + j * kTlBlock < k1) atomicAdd(&acc[x[j] >> ts.sb], cx[j]);
             }
+#pragma unroll
+            for (int j = 0; j < 4; j++) w[j] = wn[j];
             k0 = k1;
+            k1 = k2;
         }
         __syncthreads();
         const uint64_t v0 = d << ts.db;
