@@ -145,8 +145,11 @@ int graph_upload(asc_graph* g) {
             g->pk_wmin = wlo;
             g->part_edges = std::max<uint64_t>(pb / 4, 1024);
         } else if (wbits <= 8) {
-            ASC_TRY(g->ptail.alloc_prefer_pinned(n, ctx->cfg.max_pinned_bytes));
-            ASC_TRY(g->ptail_w.alloc_prefer_pinned(n, ctx->cfg.max_pinned_bytes));
+            // ids and weight bytes are page-locked together (5 bytes an edge
+            // against the cap) or not at all
+            const uint64_t cap = ctx->cfg.max_pinned_bytes;
+            ASC_TRY(!cap || 5 * n <= cap ? g->ptail.alloc_prefer_pinned(n, 0) : g->ptail.alloc(n, false));
+            ASC_TRY(g->ptail.pinned ? g->ptail_w.alloc_prefer_pinned(n, 0) : g->ptail_w.alloc(n, false));
             host_pool().parallel_for(n, [&](uint64_t b, uint64_t en) {
                 for (uint64_t i = b; i < en; i++) {
                     g->ptail[i] = ew[2 * (S + i)];

@@ -518,8 +518,15 @@ int graph_generate(asc_ctx* ctx, const asc_gen_params* p, asc_graph** out) {
     }
     g->device_bytes = ob * (V + 1) + sbytes;
     if (n_tail) {
-        ASC_TRY(g->ptail.alloc_prefer_pinned(n_tail, ctx->cfg.max_pinned_bytes));
-        if (split) ASC_TRY(g->ptail_w.alloc_prefer_pinned(n_tail, ctx->cfg.max_pinned_bytes));
+        // split form: ids and weight bytes page-locked together (5 bytes an
+        // edge against the cap) or not at all
+        const uint64_t cap = ctx->cfg.max_pinned_bytes;
+        if (split) {
+            ASC_TRY(!cap || 5 * n_tail <= cap ? g->ptail.alloc_prefer_pinned(n_tail, 0) : g->ptail.alloc(n_tail, false));
+            ASC_TRY(g->ptail.pinned ? g->ptail_w.alloc_prefer_pinned(n_tail, 0) : g->ptail_w.alloc(n_tail, false));
+        } else {
+            ASC_TRY(g->ptail.alloc_prefer_pinned(n_tail, cap));
+        }
         const uint64_t pb = ctx->cfg.part_bytes ? ctx->cfg.part_bytes : (uint64_t(64) << 20);
         g->packed = true;
         g->pk_split = split;
