@@ -337,7 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_do_fill(uint32_t* p, uint64_t n, uin
 // End of a level: the level's counters (and the next top-down frontier's
 // size, in `pad`) are written to mapped host memory, and the device counters
 // are cleared for the next level (no copy or memset dispatch per level).
-__global__ void k_do_publish(DoCounters* c, const Slot* next_slot, DoCounters* host) {
+__global__ void k_do_publish(DoCounters* c, const Slot* next_slot, DoCounters* host, unsigned long long seq) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const DoCounters v = *c;
     host->n_found = v.n_found;
@@ -345,6 +345,7 @@ __global__ void k_do_publish(DoCounters* c, const Slot* next_slot, DoCounters* h
     host->m_frontier = v.m_frontier;
     host->pad = next_slot ? next_slot->n_res : 0;
     __threadfence_system();
+    host[1].n_found = seq;   // the counters above are complete (the host polls this word)
     c->n_found = 0;
     c->m_found = 0;
     c->m_frontier = 0;
@@ -530,10 +531,34 @@ static int ensure_fast_state(asc_graph* g) {
     }
     if (!t->dcnt) {
         ASC_HIP(hipMalloc(&t->dcnt, sizeof(DoCounters)));
-        ASC_TRY(t->h_dcnt.alloc(1, true));
+        ASC_TRY(t->h_dcnt.alloc(2, true));   // [0] counters, [1].n_found the publish sequence
+        t->h_dcnt[1].n_found = 0;
+        t->pub_seq = 0;
         ASC_HIP(hipHostGetDevicePointer((void**)&t->d_hdcnt, t->h_dcnt.p, 0));
         ASC_TRY(t->h_sample.alloc(kSamples, true));
     }
+    return ASC_OK;
+}
+
+// Waits until k_do_publish has written sequence number `want` to mapped host
+// memory, i.e. the level's kernels have ended and their counters are there,
+// by polling that word instead of synchronizing the stream (which adds tens
+// of microseconds a level).  After 50 ms of polling it synchronizes the stream
+// instead, which also reports a failed launch.
+static int wait_publish(TravState* t, hipStream_t s, unsigned long long want) {
+    const volatile unsigned long long* seq = &t->h_dcnt.p[1].n_found;
+    const double t0 = now_ms();
+    for (uint32_t i = 0; *seq != want; i++) {
+        if ((i & 1023) == 1023 && now_ms() - t0 > 50.0) {
+            ASC_HIP(hipStreamSynchronize(s));
+            if (*seq != want) {
+                set_last_error("the BFS level counters were not published");
+                return ASC_ERR_DEVICE;
+            }
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
     return ASC_OK;
 }
 
@@ -633,10 +658,11 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             ASC_HIP(hipEventRecord(e1, s));
             t->timed.push_back({-1, e0, e1});
         }
+        const unsigned long long want = ++t->pub_seq;
         hipLaunchKernelGGL(k_do_publish, dim3(1), dim3(64), 0, s, t->dcnt,
-                           bottom_up ? (const Slot*)nullptr : t->slots + ((level + 1) & 3), t->d_hdcnt);
+                           bottom_up ? (const Slot*)nullptr : t->slots + ((level + 1) & 3), t->d_hdcnt, want);
         ASC_HIP(hipGetLastError());
-        ASC_HIP(hipStreamSynchronize(s));
+        ASC_TRY(wait_publish(t, s, want));
         prev_nf = nf;
         const volatile DoCounters* pub = t->h_dcnt.p;   // written by k_do_publish
         if (bottom_up) {

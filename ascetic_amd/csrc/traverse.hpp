@@ -182,8 +182,9 @@ struct TravState {
     uint32_t* bm[3] = {nullptr, nullptr, nullptr};
     uint32_t* pctl = nullptr;
     DoCounters* dcnt = nullptr;
-    HostBuf<DoCounters> h_dcnt;     // written by the device through d_hdcnt (mapped)
-    DoCounters* d_hdcnt = nullptr;
+    HostBuf<DoCounters> h_dcnt;     // written by the device through d_hdcnt (mapped): [0] the
+    DoCounters* d_hdcnt = nullptr;  // counters, [1].n_found the sequence number of the last publish
+    unsigned long long pub_seq = 0;
     HostBuf<uint32_t> h_sample;
     // device part buffer b: copy into it done / last kernel reading it done;
     // staging buffer i: copy out of it done
