@@ -540,28 +540,6 @@ static int ensure_fast_state(asc_graph* g) {
     return ASC_OK;
 }
 
-// Waits until k_do_publish has written sequence number `want` to mapped host
-// memory, i.e. the level's kernels have ended and their counters are there,
-// by polling that word instead of synchronizing the stream (which adds tens
-// of microseconds a level).  After 50 ms of polling it synchronizes the stream
-// instead, which also reports a failed launch.
-static int wait_publish(TravState* t, hipStream_t s, unsigned long long want) {
-    const volatile unsigned long long* seq = &t->h_dcnt.p[1].n_found;
-    const double t0 = now_ms();
-    for (uint32_t i = 0; *seq != want; i++) {
-        if ((i & 1023) == 1023 && now_ms() - t0 > 50.0) {
-            ASC_HIP(hipStreamSynchronize(s));
-            if (*seq != want) {
-                set_last_error("the BFS level counters were not published");
-                return ASC_ERR_DEVICE;
-            }
-            break;
-        }
-    }
-    std::atomic_thread_fence(std::memory_order_acquire);
-    return ASC_OK;
-}
-
 template <class OffT, class EdgeT>
 static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) {
     asc_ctx* ctx = g->ctx;
@@ -662,7 +640,7 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
         hipLaunchKernelGGL(k_do_publish, dim3(1), dim3(64), 0, s, t->dcnt,
                            bottom_up ? (const Slot*)nullptr : t->slots + ((level + 1) & 3), t->d_hdcnt, want);
         ASC_HIP(hipGetLastError());
-        ASC_TRY(wait_publish(t, s, want));
+        ASC_TRY(wait_host_seq(&t->h_dcnt.p[1].n_found, want, s));
         prev_nf = nf;
         const volatile DoCounters* pub = t->h_dcnt.p;   // written by k_do_publish
         if (bottom_up) {

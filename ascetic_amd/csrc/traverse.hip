@@ -410,6 +410,16 @@ __global__ __launch_bounds__(kBlock) void k_split_far(TravArgs<OffT, EdgeT> a, c
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
+// Publishes a level's counters (the next step's slot and the piles) to mapped
+// host memory, then the sequence number the host polls for.
+__global__ void k_trav_publish(const Slot* slot, const Piles* piles, LevelPub* host, unsigned long long seq) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    host->slot = *slot;
+    host->piles = *piles;
+    __threadfence_system();
+    host->seq = seq;
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -552,6 +562,9 @@ int trav_ensure(asc_graph* g) {
     ASC_HIP(hipMalloc(&t->stats, sizeof(DevStats)));
     ASC_TRY(t->h_slot.alloc(4, true));
     ASC_TRY(t->h_piles.alloc(1, true));
+    ASC_TRY(t->h_pub.alloc(1, true));
+    t->h_pub[0].seq = 0;
+    ASC_HIP(hipHostGetDevicePointer((void**)&t->d_pub, t->h_pub.p, 0));
     ASC_TRY(t->h_stats.alloc(1, true));
     if (need_parts) {
         const uint64_t nod = V - g->static_vertices;
@@ -1301,10 +1314,23 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
             if (nstream) ASC_TRY((stream_pile<ALGO, OffT, EdgeT>(g, a, level, stream_list, nstream, grid, rc)));
             level++;
         }
-        ASC_HIP(hipMemcpyAsync(t->h_slot.p, t->slots + (level & 3), sizeof(Slot), hipMemcpyDeviceToHost, s));
-        ASC_HIP(hipMemcpyAsync(t->h_piles.p, t->piles, sizeof(Piles), hipMemcpyDeviceToHost, s));
-        ASC_HIP(hipStreamSynchronize(s));
-        nr = t->h_slot[0].n_res;
+        const unsigned long long want = ++t->lvl_seq;
+        hipLaunchKernelGGL(k_trav_publish, dim3(1), dim3(1), 0, s, t->slots + (level & 3), t->piles, t->d_pub, want);
+        ASC_HIP(hipGetLastError());
+        ASC_TRY(wait_host_seq(&t->h_pub.p->seq, want, s));
+        {
+            const volatile LevelPub* pb = t->h_pub.p;
+            nr = pb->slot.n_res;
+            Piles pl;
+            pl.n_pend[0] = pb->piles.n_pend[0];
+            pl.n_pend[1] = pb->piles.n_pend[1];
+            pl.n_far[0] = pb->piles.n_far[0];
+            pl.n_far[1] = pb->piles.n_far[1];
+            pl.n_list = pb->piles.n_list;
+            pl.min_far = pb->piles.min_far;
+            pl.pad[0] = pl.pad[1] = 0;
+            t->h_piles[0] = pl;   // min_far is read when the band advances
+        }
         npend = t->h_piles[0].n_pend[pcur];
         nfar = ALGO == ALG_SSSP ? t->h_piles[0].n_far[fcur] : 0;
         last_split_empty = mode == M_SPLIT && nr == 0 && npend == 0;

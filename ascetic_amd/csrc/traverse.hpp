@@ -122,6 +122,35 @@ struct RunCounters {
     double t_start = 0;             // host clock at the start of the run (per-round log)
 };
 
+// A level's counters as the device publishes them to mapped host memory; seq
+// is written last, so a host that reads the expected seq finds the rest final.
+struct LevelPub {
+    Slot slot;
+    Piles piles;
+    unsigned long long seq;
+};
+
+// Waits until the device has written `want` to *seq (mapped host memory),
+// i.e. the kernels before the publishing one have ended, by polling instead
+// of synchronizing the stream (which adds tens of microseconds).  After 50 ms
+// of polling it synchronizes the stream instead, which also reports a failed
+// launch.
+inline int wait_host_seq(const volatile unsigned long long* seq, unsigned long long want, hipStream_t s) {
+    const double t0 = now_ms();
+    for (uint32_t i = 0; *seq != want; i++) {
+        if ((i & 1023) == 1023 && now_ms() - t0 > 50.0) {
+            ASC_HIP(hipStreamSynchronize(s));
+            if (*seq != want) {
+                set_last_error("the device did not publish its level counters");
+                return ASC_ERR_DEVICE;
+            }
+            break;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return ASC_OK;
+}
+
 struct TravState {
     uint64_t V = 0;
     uint32_t* val = nullptr;
@@ -153,6 +182,9 @@ struct TravState {
     uint64_t part_edges = 0;
     HostBuf<Slot> h_slot;
     HostBuf<Piles> h_piles;
+    HostBuf<LevelPub> h_pub;        // written by the device through d_pub (mapped)
+    LevelPub* d_pub = nullptr;
+    unsigned long long lvl_seq = 0;
     HostBuf<DevStats> h_stats;
     // the host gather's index of a round's sparse segment: first record of
     // each listed vertex in the gathered array (written by the device through
