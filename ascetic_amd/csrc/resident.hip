@@ -316,20 +316,6 @@ __global__ __launch_bounds__(kBlock) void k_bitmap_to_queue(const uint32_t* bm, 
     }
 }
 
-// Summed degrees of a frontier queue (count in slot->n_res).
-template <class OffT>
-__global__ __launch_bounds__(kBlock) void k_frontier_edges(const OffT* off, const uint32_t* q, const Slot* slot,
-                                                           unsigned long long* out) {
-    __shared__ unsigned long long s_tmp[kWavesPerBlock];
-    const uint32_t n = slot->n_res;
-    unsigned long long m = 0;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-        const uint32_t v = q[i];
-        m += (uint64_t)(off[v + 1] - off[v]);
-    }
-    block_add(m, s_tmp, out);
-}
-
 __global__ __launch_bounds__(kBlock) void k_do_fill(uint32_t* p, uint64_t n, uint32_t v) {
     for (uint64_t i = blockIdx.x * (uint64_t)kBlock + threadIdx.x; i < n; i += (uint64_t)gridDim.x * kBlock) p[i] = v;
 }
@@ -576,6 +562,9 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
     a.vs = (uint32_t)V;   // every vertex is resident: nothing goes to a pile
     a.heavy_thr = (uint32_t)effective_heavy_threshold(g);
     a.theta = 0xFFFFFFFFu;
+    // top-down levels add the degrees of the vertices they claim here: the
+    // next frontier's edges, for the direction rule
+    a.m_next = &t->dcnt->m_frontier;
 
     RunCounters rc{};
     t->timed.clear();
@@ -626,9 +615,6 @@ static int run_bfs_do(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* st) 
             ASC_TRY(timed_launch(t, prof, KERN_HEAVY, s, [&] {
                 hipLaunchKernelGGL((k_expand_heavy<ALG_BFS, OffT, EdgeT>), dim3(grid), dim3(kBlock), 0, s, a, level);
             }));
-            hipLaunchKernelGGL((k_frontier_edges<OffT>), dim3(grid), dim3(kBlock), 0, s, a.off, a.q_out,
-                               t->slots + ((level + 1) & 3), &t->dcnt->m_frontier);
-            ASC_HIP(hipGetLastError());
         }
         if (!prof) {
             hipEvent_t e1 = next_event(t);
