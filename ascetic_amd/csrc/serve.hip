@@ -10,7 +10,9 @@
 //         lev[v][q] = hops + 1 up to 254 (a row of 64 lanes is one 64-byte
 //         line, so the lanes that reach a vertex in one step write one line);
 //         a lane whose levels go beyond is answered by the single-query
-//         engine when it leaves its lane.
+//         engine when it leaves its lane.  lev[v][q] is meaningful only
+//         where bit q of seen[v] is set (it is written when the bit is), so
+//         an admitted lane clears its seen bits and leaves its level bytes.
 //   SSSP  distance lanes: dist[v][q] vertex-major in 16 bits (a row of 64
 //         lanes is one 128-byte line), one thread per edge relaxing every
 //         lane whose frontier holds the edge's source; improved lanes are
@@ -70,7 +72,7 @@ struct BatchArgs {
     unsigned long long* X;           // frontier bits of this step (read and cleared)
     unsigned long long* Y;           // frontier bits of the next step
     unsigned long long* seen;        // BFS
-    uint8_t* lev;                    // BFS: lev[v * stride + q], hops + 1, 0xFF = unreached
+    uint8_t* lev;                    // BFS: lev[v * stride + q], hops + 1, where seen[v] has bit q
     uint16_t* dist;                  // SSSP: dist[v * stride + q], 0xFFFF = unreached
     unsigned long long* ovf;         // lanes whose levels / distances left 8 / 16 bits
     const uint32_t* q_in;
@@ -684,9 +686,9 @@ __global__ void k_clear_seen_bits(unsigned long long* seen, uint64_t V, unsigned
         seen[v] &= ~mask;
 }
 
-// Lanes of a state row set to all ones (BFS level 0xFF, SSSP distance
-// 0xFFFF: unreached): `m` has the lanes' bytes set, rows are nu4 16-byte units
-// (at most 8); each thread ORs whole units of one row.
+// Lanes of an SSSP distance row set to all ones (0xFFFF: unreached): `m`
+// has the lanes' bytes set, rows are nu4 16-byte units (at most 8); each
+// thread ORs whole units of one row.
 struct RowMask {
     uint4 m[8];
 };
@@ -725,12 +727,12 @@ __global__ void k_seed(BatchArgs<OffT, EdgeT> a, SeedArgs sa, uint32_t step, uin
     }
 }
 
-// Result column of one BFS lane, unreached as V + 1.
-__global__ void k_extract_lev(const uint8_t* lev, uint64_t V, uint32_t stride, uint32_t lane, uint32_t* out) {
-    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x) {
-        const uint32_t l = lev[v * stride + lane];
-        out[v] = l == 0xFFu ? (uint32_t)(V + 1) : l;
-    }
+// Result column of one BFS lane, unreached (seen bit clear) as V + 1; the
+// level bytes of vertices the lane did not reach may hold an earlier query's.
+__global__ void k_extract_lev(const uint8_t* lev, const unsigned long long* seen, uint64_t V, uint32_t stride,
+                              uint32_t lane, uint32_t* out) {
+    for (uint64_t v = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; v < V; v += (uint64_t)gridDim.x * blockDim.x)
+        out[v] = ((seen[v] >> lane) & 1ull) ? (uint32_t)lev[v * stride + lane] : (uint32_t)(V + 1);
 }
 
 __global__ void k_lane_fill(uint32_t* p, uint64_t n, uint32_t v) {
@@ -995,11 +997,12 @@ static int engine_launch(asc_graph* g, Engine& e, const std::vector<SQuery*>& ad
             mask |= 1ull << q->lane;
         }
         if (e.kind == ASC_QUERY_BFS) {
+            // unreached everywhere; the level bytes are read only where a seen
+            // bit is set, so they keep whatever an earlier query left
             hipLaunchKernelGGL(k_clear_seen_bits, dim3(grid), dim3(256), 0, e.stream, e.seen, e.V, mask);
-        }
-        {
-            // the admitted lanes' state: every vertex unreached
-            const uint32_t lane_bytes = e.kind == ASC_QUERY_BFS ? 1 : 2;
+        } else {
+            // the admitted lanes' distances: every vertex unreached
+            const uint32_t lane_bytes = 2;
             const uint32_t row_bytes = e.stride * lane_bytes;
             RowMask rm{};
             uint8_t* mb = (uint8_t*)rm.m;
@@ -1081,7 +1084,7 @@ static int engine_finish(asc_server* s, Engine& e, std::vector<SQuery*>& done) {
             const int grid = s->g->ctx->num_cus * 4;
             if (e.kind == ASC_QUERY_BFS)
                 hipLaunchKernelGGL(k_extract_lev, dim3(grid), dim3(256), 0, e.stream, (const uint8_t*)e.lanes_state,
-                                   e.V, e.stride, (uint32_t)q, e.staging);
+                                   (const unsigned long long*)e.seen, e.V, e.stride, (uint32_t)q, e.staging);
             else
                 hipLaunchKernelGGL(k_extract_lane, dim3(grid), dim3(256), 0, e.stream,
                                    (const uint16_t*)e.lanes_state, e.V, e.stride, (uint32_t)q, e.staging);
