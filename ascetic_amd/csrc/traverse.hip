@@ -371,7 +371,11 @@ __global__ __launch_bounds__(kBlock) void k_scan_apply(const uint32_t* x, uint32
 }
 
 // Moves the far pile's entries whose value fell below theta into the current
-// frontier (resident) or the on-demand pile; the rest stays far.
+// frontier (resident) or the on-demand pile; the rest stays far.  A vertex
+// enters the far pile once per run (its far stamp stays set): a value that
+// falls below theta is expanded in the band and can only fall further, and a
+// value that stays above is read again from the one entry at the next split,
+// so kept entries are copied without a stamp atomic.
 template <class OffT, class EdgeT>
 __global__ __launch_bounds__(kBlock) void k_split_far(TravArgs<OffT, EdgeT> a, const uint32_t* far_in, uint32_t n,
                                                       uint32_t level, uint32_t* q_level, Piles* piles) {
@@ -390,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_split_far(TravArgs<OffT, EdgeT> a, c
                 if (d < a.theta) {
                     if (v < a.vs) to = atomicMax(a.stamp + v, level) < level ? TO_RES : TO_NONE;
                     else to = atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
-                } else if (atomicMax(a.fstamp + v, a.fepoch) < a.fepoch) {
+                } else {
                     to = TO_FAR;
                     lmin = min(lmin, d);
                 }
@@ -1273,7 +1277,6 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
             a.theta = (uint32_t)std::min<uint64_t>(th, 0xFFFFFFFFull);
             const uint32_t old = fcur;
             fcur ^= 1;
-            a.fepoch++;
             a.far_out = t->far[fcur];
             a.far_count = &t->piles->n_far[fcur];
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, s, &t->piles->n_far[fcur], 0u);

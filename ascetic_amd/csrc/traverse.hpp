@@ -74,7 +74,7 @@ struct TravArgs {
     uint32_t* val;
     uint32_t* stamp;                // SSSP/CC: level a resident vertex was last queued for
     uint32_t* pstamp;               // SSSP/CC: on-demand pile epoch a vertex was last added to
-    uint32_t* fstamp;               // SSSP: far pile epoch a vertex was last added to
+    uint32_t* fstamp;               // SSSP: set (to fepoch, constant in a run) once a vertex enters the far pile
     uint32_t* done;                 // SSSP/CC: value a vertex was last expanded with
     uint32_t* visited;              // BFS bitmap
     const uint32_t* q_in;
