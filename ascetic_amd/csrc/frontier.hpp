@@ -23,11 +23,17 @@ __device__ __forceinline__ uint32_t candidate(uint32_t sv, uint32_t w) {
 // Check-then-atomicMin relaxation (gpu_kernels.cuh:390-392, 499-502, 453-456)
 // and routing of an improved vertex, in two steps so that a wave can issue
 // the probes of several edges before any of their atomics: relax_probe loads
-// the destination's word (BFS: visited bits; SSSP/CC: value),
+// the destination's word (BFS: visited bits; SSSP: its value filter; CC:
+// value),
 // relax_commit_batch decides from it and performs the atomics.
 template <int ALGO, class OffT, class EdgeT>
 __device__ __forceinline__ uint32_t relax_probe(const TravArgs<OffT, EdgeT>& a, uint32_t v) {
-    return ALGO == ALG_BFS ? a.visited[v >> 5] : a.val[v];
+    if (ALGO == ALG_BFS) return a.visited[v >> 5];
+    if (ALGO == ALG_SSSP && a.val16) {
+        const uint32_t f = a.val16[v];
+        return f == 0xFFFFu ? 0xFFFFFFFFu : f;
+    }
+    return a.val[v];
 }
 
 // Edges each lane takes per pass of an edge loop: the loads of kUnroll edges,
@@ -74,6 +80,7 @@ __device__ __forceinline__ void relax_commit_batch(const TravArgs<OffT, EdgeT>& 
     for (int j = 0; j < kUnroll; j++) {
         kind[j] = TO_NONE;
         if (!try_[j] || old[j] <= cand[j]) continue;
+        if (ALGO == ALG_SSSP && a.val16 && cand[j] < 0xFFFFu) a.val16[v[j]] = (uint16_t)cand[j];
         if (ALGO == ALG_SSSP && cand[j] >= a.theta) {
             sp[j] = a.fstamp + v[j];
             sv[j] = a.fepoch;

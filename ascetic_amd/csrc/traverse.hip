@@ -390,14 +390,14 @@ __global__ __launch_bounds__(kBlock) void k_split_far(TravArgs<OffT, EdgeT> a, c
         if (i < n) {
             v = far_in[i];
             const uint32_t d = a.val[v];
-            if (a.done[v] != d) {
-                if (d < a.theta) {
-                    if (v < a.vs) to = atomicMax(a.stamp + v, level) < level ? TO_RES : TO_NONE;
-                    else to = atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
-                } else {
-                    to = TO_FAR;
-                    lmin = min(lmin, d);
-                }
+            if (d >= a.theta) {
+                // a vertex is expanded only below the band of its round and
+                // bands only rise, so done[v] < theta here: still pending
+                to = TO_FAR;
+                lmin = min(lmin, d);
+            } else if (a.done[v] != d) {
+                if (v < a.vs) to = atomicMax(a.stamp + v, level) < level ? TO_RES : TO_NONE;
+                else to = atomicMax(a.pstamp + v, a.pepoch) < a.pepoch ? TO_PEND : TO_NONE;
             }
         }
         pq.res.push(to == TO_RES, v, q_level, &slot->n_res);
@@ -471,7 +471,7 @@ void trav_free(TravState* t) {
     auto f = [](void* p) {
         if (p) (void)hipFree(p);
     };
-    f(t->val); f(t->stamp); f(t->pstamp); f(t->fstamp); f(t->done); f(t->visited);
+    f(t->val); f(t->val16); f(t->stamp); f(t->pstamp); f(t->fstamp); f(t->done); f(t->visited);
     for (int i = 0; i < 2; i++) {
         f(t->q[i]); f(t->pend[i]); f(t->far[i]);
     }
@@ -1198,6 +1198,14 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
                                t->slots, t->piles, (uint32_t)V, vs);
         } else {
             hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, t->val, V, (uint32_t)(V + 1));
+            if (ALGO == ALG_SSSP && vs == V) {
+                // the value filter serves fully resident graphs (narrow bands:
+                // few improvements per vertex); with on-demand edges (wide
+                // bands, more stores to it) it measured slower
+                if (!t->val16) ASC_HIP(hipMalloc(&t->val16, 4 * ((V + 1) / 2)));
+                hipLaunchKernelGGL(k_fill_u32, dim3(fill_grid), dim3(256), 0, s, (uint32_t*)t->val16, (V + 1) / 2,
+                                   0xFFFFFFFFu);
+            }
             hipLaunchKernelGGL(k_init_source, dim3(1), dim3(1), 0, s, t->val, (uint32_t*)nullptr, t->pstamp, t->q[0],
                                t->pend[0], t->slots, t->piles, src, vs);
         }
@@ -1209,6 +1217,7 @@ static int run_traversal(asc_graph* g, uint32_t src, uint32_t* out, asc_stats* s
     a.off = (const OffT*)g->d_off;
     a.edges = (const EdgeT*)g->d_edges;
     a.val = t->val;
+    a.val16 = ALGO == ALG_SSSP && vs == V ? t->val16 : nullptr;
     a.stamp = t->stamp;
     a.pstamp = t->pstamp;
     a.fstamp = t->fstamp;

@@ -72,6 +72,13 @@ struct TravArgs {
     const OffT* off;                // V+1 offsets
     const EdgeT* edges;             // static edge prefix
     uint32_t* val;
+    // SSSP on a fully resident graph (else null): 16-bit value filter
+    // probed instead of val.  Each entry is 0xFFFF
+    // (nothing known) or a value val[v] held at some moment (stored after the
+    // atomicMin that set it, so a late store can leave a larger one): never
+    // below val[v], so a candidate not below it cannot lower val[v].  Half
+    // the bytes of val, so more of the probed vertices stay in cache.
+    uint16_t* val16;
     uint32_t* stamp;                // SSSP/CC: level a resident vertex was last queued for
     uint32_t* pstamp;               // SSSP/CC: on-demand pile epoch a vertex was last added to
     uint32_t* fstamp;               // SSSP: set (to fepoch, constant in a run) once a vertex enters the far pile
@@ -154,6 +161,7 @@ inline int wait_host_seq(const volatile unsigned long long* seq, unsigned long l
 struct TravState {
     uint64_t V = 0;
     uint32_t* val = nullptr;
+    uint16_t* val16 = nullptr;        // SSSP value filter (TravArgs::val16)
     uint32_t* stamp = nullptr;
     uint32_t* pstamp = nullptr;
     uint32_t* fstamp = nullptr;
