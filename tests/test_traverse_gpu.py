@@ -195,6 +195,28 @@ def test_everything_on_demand(gpu_ctx, mode, pack):
             assert np.array_equal(lv, O.bfs_queue(csr, 5))
 
 
+@pytest.mark.parametrize("wmax", [20000, 1 << 20])
+def test_resident_sssp_values_beyond_16_bits(gpu_ctx, wmax):
+    """Resident SSSP probes a 16-bit copy of the values (0xFFFF: nothing
+    known) before its atomics.  With 100000 vertices a value can exceed 65535
+    (values are capped at V + 1): weights up to 20000 put the distances on
+    both sides of it, weights up to 2^20 leave most candidates above the cap."""
+    say("resident sssp, weights up to", wmax)
+    off, dst, _ = G.random_graph(100000, 250000, 11, symmetric=True)
+    V = len(off) - 1
+    w = np.random.default_rng(12).integers(1, wmax + 1, int(off[-1])).astype(np.uint32)
+    csr = O.CSR(off, dst, w)
+    with make_graph(gpu_ctx, off, dst, w) as g:
+        assert g.info()["static_vertices"] == V
+        for s in (0, int(np.argmax(np.diff(off)))):
+            ref = O.dijkstra(csr, s)
+            if wmax == 20000:
+                reached = ref[ref != V + 1]
+                assert (reached < 0xFFFF).any() and (reached > 0xFFFF).any()
+            d, _ = g.sssp(s)
+            assert np.array_equal(d, ref), s
+
+
 def test_value_conventions():
     off, dst, _ = G.two_components(5, 3)
     with A.Context(value_convention=_abi.VALUES_ZERO_BASED) as ctx:
