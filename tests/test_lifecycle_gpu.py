@@ -47,6 +47,25 @@ def test_replan_that_cannot_fit_keeps_the_graph():
                 assert g2.info()["static_edges"] == before["static_edges"]
 
 
+def test_sssp_across_replans_resident_and_streamed():
+    """SSSP on one graph streamed, re-planned fully resident (where it probes
+    a 16-bit value filter), then streamed again: the oracle's values each
+    time, twice per plan (the traversal state is reused between runs)."""
+    off, dst, w = _graph()
+    E = int(off[-1])
+    csr = O.CSR(off, dst, w)
+    refs = {s: O.dijkstra(csr, s) for s in (0, 3)}
+    with A.Context(edge_budget_bytes=8 * E // 3) as ctx, ctx.from_arrays(off, G.wedges(dst, w)) as g:
+        for budget in (8 * E // 3, 8 * E, 8 * E // 4):
+            g.replan(edge_budget_bytes=budget)
+            resident = g.info()["static_edges"] == E
+            assert resident == (budget == 8 * E)
+            for s, ref in refs.items():
+                d, st = g.sssp(s)
+                assert np.array_equal(d, ref), (budget, s)
+                assert (st["streamed_edges"] == 0) or not resident
+
+
 def test_replan_and_destroy_refused_while_a_server_runs():
     off, dst, w = _graph()
     csr = O.CSR(off, dst, w)
