@@ -30,8 +30,10 @@ template <int ALGO, class OffT, class EdgeT>
 __device__ __forceinline__ uint32_t relax_probe(const TravArgs<OffT, EdgeT>& a, uint32_t v) {
     if (ALGO == ALG_BFS) return a.visited[v >> 5];
     if (ALGO == ALG_SSSP && a.val16) {
+        // 0xFFFF: nothing known (not lowered yet, or a value beyond 16 bits),
+        // so the value itself is read
         const uint32_t f = a.val16[v];
-        return f == 0xFFFFu ? 0xFFFFFFFFu : f;
+        if (f != 0xFFFFu) return f;
     }
     return a.val[v];
 }

@@ -73,11 +73,12 @@ struct TravArgs {
     const EdgeT* edges;             // static edge prefix
     uint32_t* val;
     // SSSP on a fully resident graph (else null): 16-bit value filter
-    // probed instead of val.  Each entry is 0xFFFF (nothing known) or a
-    // value val[v] held at some moment (stored after the atomicMin that set
-    // it, so a late store can leave a larger one): never below val[v], so a
-    // candidate not below it cannot lower val[v].  Half the bytes of val, so
-    // more of the probed vertices stay in cache.
+    // probed before val.  An entry other than 0xFFFF is a value val[v] held
+    // at some moment (stored after the atomicMin that set it, so a late
+    // store can leave a larger one): never below val[v], so a candidate not
+    // below it cannot lower val[v].  0xFFFF (not lowered yet, or a value
+    // beyond 16 bits): val[v] is read.  Half the bytes of val, so more of
+    // the probed vertices stay in cache.
     uint16_t* val16;
     uint32_t* stamp;                // SSSP/CC: level a resident vertex was last queued for
     uint32_t* pstamp;               // SSSP/CC: on-demand pile epoch a vertex was last added to

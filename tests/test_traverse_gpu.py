@@ -197,8 +197,8 @@ def test_everything_on_demand(gpu_ctx, mode, pack):
 
 @pytest.mark.parametrize("wmax", [20000, 1 << 20])
 def test_resident_sssp_values_beyond_16_bits(gpu_ctx, wmax):
-    """Resident SSSP probes a 16-bit copy of the values (0xFFFF: nothing
-    known) before its atomics.  With 100000 vertices a value can exceed 65535
+    """Resident SSSP probes a 16-bit copy of the values before its atomics
+    (0xFFFF: nothing known, the value is read in full).  With 100000 vertices a value can exceed 65535
     (values are capped at V + 1): weights up to 20000 put the distances on
     both sides of it, weights up to 2^20 leave most candidates above the cap."""
     say("resident sssp, weights up to", wmax)
