@@ -31,9 +31,9 @@ __device__ __forceinline__ uint32_t relax_probe(const TravArgs<OffT, EdgeT>& a, 
     if (ALGO == ALG_BFS) return a.visited[v >> 5];
     if (ALGO == ALG_SSSP && a.val16) {
         // 0xFFFF: nothing known (not lowered yet, or a value beyond 16 bits),
-        // so the value itself is read
+        // so the atomic is tried
         const uint32_t f = a.val16[v];
-        if (f != 0xFFFFu) return f;
+        return f == 0xFFFFu ? 0xFFFFFFFFu : f;
     }
     return a.val[v];
 }

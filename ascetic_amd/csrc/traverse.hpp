@@ -77,8 +77,8 @@ struct TravArgs {
     // at some moment (stored after the atomicMin that set it, so a late
     // store can leave a larger one): never below val[v], so a candidate not
     // below it cannot lower val[v].  0xFFFF (not lowered yet, or a value
-    // beyond 16 bits): val[v] is read.  Half the bytes of val, so more of
-    // the probed vertices stay in cache.
+    // beyond 16 bits): the atomic is tried.  Half the bytes of val, so more
+    // of the probed vertices stay in cache.
     uint16_t* val16;
     uint32_t* stamp;                // SSSP/CC: level a resident vertex was last queued for
     uint32_t* pstamp;               // SSSP/CC: on-demand pile epoch a vertex was last added to

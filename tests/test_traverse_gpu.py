@@ -195,22 +195,43 @@ def test_everything_on_demand(gpu_ctx, mode, pack):
             assert np.array_equal(lv, O.bfs_queue(csr, 5))
 
 
-@pytest.mark.parametrize("wmax", [20000, 1 << 20])
-def test_resident_sssp_values_beyond_16_bits(gpu_ctx, wmax):
+def _graph_with_heavy_chain(seed):
+    """100000 vertices with 250000 random edges of weight 1-16, plus a chain
+    of 40 vertices hung from vertex 0 by edges of weight 3000: the mean weight
+    stays small while the chain's values pass 65535 (symmetric weights)."""
+    rng = np.random.default_rng(seed)
+    u = rng.integers(0, 100000, 250000)
+    v = rng.integers(0, 100000, 250000)
+    keep = u != v
+    u, v = u[keep], v[keep]
+    w = rng.integers(1, 17, len(u))
+    chain = np.arange(100000, 100040)
+    u = np.concatenate([u, [0], chain[:-1]])
+    v = np.concatenate([v, chain[:1], chain[1:]])
+    w = np.concatenate([w, np.full(40, 3000)])
+    return G.csr_from_edges(100040, zip(u.tolist(), v.tolist()), w.tolist(), symmetric=True)
+
+
+@pytest.mark.parametrize("kind,wmax", [("chain", 3000), ("random", 20000), ("random", 1 << 20)])
+def test_resident_sssp_values_beyond_16_bits(gpu_ctx, kind, wmax):
     """Resident SSSP probes a 16-bit copy of the values before its atomics
-    (0xFFFF: nothing known, the value is read in full).  With 100000 vertices a value can exceed 65535
-    (values are capped at V + 1): weights up to 20000 put the distances on
-    both sides of it, weights up to 2^20 leave most candidates above the cap."""
-    say("resident sssp, weights up to", wmax)
-    off, dst, _ = G.random_graph(100000, 250000, 11, symmetric=True)
+    (0xFFFF: nothing known, the atomic is tried) where the values should fit
+    16 bits.  With 100000 vertices a value can exceed 65535 (values are
+    capped at V + 1): the heavy chain keeps the filter on with values on both
+    sides of 65535; random weights up to 20000 or 2^20 turn it off."""
+    say("resident sssp", kind, "weights up to", wmax)
+    if kind == "chain":
+        off, dst, w = _graph_with_heavy_chain(11)
+    else:
+        off, dst, _ = G.random_graph(100000, 250000, 11, symmetric=True)
+        w = np.random.default_rng(12).integers(1, wmax + 1, int(off[-1])).astype(np.uint32)
     V = len(off) - 1
-    w = np.random.default_rng(12).integers(1, wmax + 1, int(off[-1])).astype(np.uint32)
     csr = O.CSR(off, dst, w)
     with make_graph(gpu_ctx, off, dst, w) as g:
         assert g.info()["static_vertices"] == V
         for s in (0, int(np.argmax(np.diff(off)))):
             ref = O.dijkstra(csr, s)
-            if wmax == 20000:
+            if wmax <= 20000:
                 reached = ref[ref != V + 1]
                 assert (reached < 0xFFFF).any() and (reached > 0xFFFF).any()
             d, _ = g.sssp(s)
